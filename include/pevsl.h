@@ -1,0 +1,189 @@
+/*
+ * pevsl.h -- C API of the MI355X spectrum-slicing library.
+ *
+ * Same function names, argument meaning, ownership and return codes as the
+ * reference's public header INC/pevsl.h (line numbers cited per group), so
+ * that a driver such as TESTS/Lap/LapPLanN.c builds against this header and
+ * links against libpevsl_amd.so instead of libpevsl.a.  The compute-heavy
+ * entry points (ParcsrMatvec, Kpmdos, LanDosG, ChebLanNr) run hand-written
+ * HIP kernels on the GPU; host arrays cross the boundary exactly where the
+ * reference passes host arrays.
+ *
+ * Library extensions (batched slices, precision, device buffers) are in
+ * pevsl_amd.h.
+ */
+#ifndef PEVSL_H
+#define PEVSL_H
+
+#include <stdio.h>
+
+#include "pevsl_def.h"
+#include "pevsl_struct.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- chebpol.c (INC/pevsl.h:12-14) ---------------------------------------
+ * SetPolDef: defaults (max_deg 5000, min_deg 2, damping 2, thresh_ext 0.2,
+ *            thresh_int 0.8, deg 0, intvtol 1e-9)       SRC/chebpol.c:10-19
+ * FindPol:   filter design for intv[0..1] inside [intv[2], intv[3]];
+ *            malloc's pol->mu; -1 on an invalid interval. SRC/chebpol.c:359
+ * FreePol:   frees pol->mu.                              SRC/chebpol.c:491 */
+void pEVSL_SetPolDef(pevsl_Polparams *pol);
+int pEVSL_FindPol(double *intv, pevsl_Polparams *pol);
+void pEVSL_FreePol(pevsl_Polparams *pol);
+
+/* ---- kpmdos.c (INC/pevsl.h:17-18) ----------------------------------------
+ * Chebyshev moments mu[0..Mdeg] (caller-allocated) of the spectral density
+ * and the eigenvalue-count estimate *ecnt for [intv[0], intv[1]].  All nvec
+ * random vectors advance together as columns of one block SpMM on the GPU.
+ * ngroups/groupid/gl_comm are accepted for source compatibility; with one
+ * process ngroups must be 1.                           SRC/kpmdos.c:36-197 */
+int pEVSL_Kpmdos(pevsl_Data *pevsl, int Mdeg, int damping, int nvec,
+                 double *intv, int ngroups, int groupid, MPI_Comm gl_comm,
+                 double *mu, double *ecnt);
+
+/* ---- landos.c (INC/pevsl.h:21-25) ----------------------------------------
+ * Lanczos DOS: nvec random vectors, msteps full-reorthogonalized Lanczos
+ * steps each (batched on the GPU), Gaussian-blurred Ritz values on npts
+ * points xdos/ydos (caller-allocated), count estimate *neig.
+ *                                                       SRC/landos.c:40-314
+ * SpslicerLan: equal-share cuts of the integrated DOS.  SRC/landos.c:331 */
+int pEVSL_LanDosG(pevsl_Data *pevsl, const int nvec, int msteps, int npts,
+                  double *xdos, double *ydos, double *neig, double *intv,
+                  int ngroups, int groupid, MPI_Comm gl_comm);
+void pEVSL_SpslicerLan(double *xi, double *yi, int n_int, int npts,
+                       double *sli);
+
+/* ---- lantrbnd.c (INC/pevsl.h:28-30) --------------------------------------
+ * Spectrum bounds by thick-restart Lanczos: not provided by this library;
+ * returns PEVSL_NOT_IMPLEMENT.                         SRC/lantrbnd.c:31 */
+int pEVSL_LanTrbounds(pevsl_Data *pevsl, int lanm, int maxit, double tol,
+                      pevsl_Parvec *vinit, int bndtype, double *lammin,
+                      double *lammax, FILE *fstats);
+
+/* ---- pevsl.c (INC/pevsl.h:40-51) -----------------------------------------
+ * Start seeds libc rand() with the rank (0) as SRC/pevsl.c:36 does, creates
+ * the HIP stream and device context.  SetAParcsr copies the CSR to HBM and
+ * installs the device matvec (SRC/pevsl.c:81-97).  SetAMatvec installs a
+ * host callback (every product then crosses PCIe).  The generalized-problem
+ * setters (B, B-solve, L^T-solve) return PEVSL_NOT_IMPLEMENT. */
+int pEVSL_Start(MPI_Comm comm, pevsl_Data **data);
+int pEVSL_Finish(pevsl_Data *pevsl_data);
+int pEVSL_SetProbSizes(pevsl_Data *pevsl_data, int N, int n, int nfirst);
+int pEVSL_SetAParcsr(pevsl_Data *pevsl_data, pevsl_Parcsr *A);
+int pEVSL_SetBParcsr(pevsl_Data *pevsl_data, pevsl_Parcsr *B);
+int pEVSL_SetAMatvec(pevsl_Data *pevsl_data, MVFunc func, void *data);
+int pEVSL_SetBMatvec(pevsl_Data *pevsl_data, MVFunc func, void *data);
+int pEVSL_SetBSol(pevsl_Data *pevsl_data, SVFunc func, void *data);
+int pEVSL_SetLTSol(pevsl_Data *pevsl_data, SVFunc func, void *data);
+int pEVSL_SetStdEig(pevsl_Data *pevsl_data);
+int pEVSL_SetGenEig(pevsl_Data *pevsl_data);
+int pEVSL_SetSigmaMult(pevsl_Data *pevsl_data, double mult);
+
+/* ---- parcsr.c (INC/pevsl.h:54-61) ----------------------------------------
+ * One rank: row_starts/col_starts must be NULL or {0, n}; Setup keeps the
+ * whole matrix in diag (offd has zero columns).        SRC/parcsr.c:24-304 */
+int pEVSL_ParcsrCreate(int nrow, int ncol, int *row_starts, int *col_starts,
+                       pevsl_Parcsr *A, MPI_Comm comm);
+int pEVSL_ParcsrSetup(pevsl_Csr *Ai, pevsl_Parcsr *A);
+void pEVSL_ParcsrFree(pevsl_Parcsr *A);
+int pEVSL_ParcsrGetLocalMat(pevsl_Parcsr *A, int idx, pevsl_Coo *coo,
+                            pevsl_Csr *csr, char stype);
+int pEVSL_ParcsrNnz(pevsl_Parcsr *A);
+int pEVSL_ParcsrLocalNnz(pevsl_Parcsr *A);
+
+/* ---- parcsrmv.c (INC/pevsl.h:64) -----------------------------------------
+ * y = A x with host vectors; the product runs on the GPU.
+ *                                                   SRC/parcsrmv.c:84-93 */
+void pEVSL_ParcsrMatvec(pevsl_Parcsr *A, pevsl_Parvec *x, pevsl_Parvec *y);
+
+/* ---- parvec.c (INC/pevsl.h:67-83): host vector utilities ----------------- */
+void pEVSL_ParvecCreate(int nglobal, int nlocal, int nfirst, MPI_Comm comm,
+                        pevsl_Parvec *x);
+void pEVSL_ParvecCreateShell(int nglobal, int nlocal, int nfirst,
+                             MPI_Comm comm, pevsl_Parvec *x, double *data);
+void pEVSL_ParvecDupl(pevsl_Parvec *x, pevsl_Parvec *y);
+void pEVSL_ParvecFree(pevsl_Parvec *x);
+void pEVSL_ParvecRand(pevsl_Parvec *x);
+void pEVSL_ParvecDot(pevsl_Parvec *x, pevsl_Parvec *y, double *t);
+void pEVSL_ParvecNrm2(pevsl_Parvec *x, double *t);
+void pEVSL_ParvecCopy(pevsl_Parvec *x, pevsl_Parvec *y);
+void pEVSL_ParvecSum(pevsl_Parvec *x, double *t);
+void pEVSL_ParvecScal(pevsl_Parvec *x, double t);
+void pEVSL_ParvecSetScalar(pevsl_Parvec *x, double t);
+void pEVSL_ParvecSetZero(pevsl_Parvec *x);
+void pEVSL_ParvecAxpy(double a, pevsl_Parvec *x, pevsl_Parvec *y);
+int pEVSL_ParvecSameSize(pevsl_Parvec *x, pevsl_Parvec *y);
+int pEVSL_ParvecWrite(pevsl_Parvec *x, const char *fn);
+
+/* ---- parvecs.c (INC/pevsl.h:86-105): host multi-vectors ------------------ */
+void pEVSL_ParvecsCreateShell(int nglobal, int nvecs, int ld, int nlocal,
+                              int nfirst, MPI_Comm comm, pevsl_Parvecs *x,
+                              double *data);
+void pEVSL_ParvecsCreate(int nglobal, int nvecs, int ld, int nlocal,
+                         int nfirst, MPI_Comm comm, pevsl_Parvecs *x);
+void pEVSL_ParvecsDuplParvec(pevsl_Parvec *x, int nvecs, int ld,
+                             pevsl_Parvecs *y);
+void pEVSL_ParvecsFree(pevsl_Parvecs *x);
+void pEVSL_ParvecsGetParvecShell(pevsl_Parvecs *X, int i, pevsl_Parvec *x);
+void pEVSL_ParvecsGemv(double alp, pevsl_Parvecs *A, int nvecs, double *x,
+                       double bet, pevsl_Parvec *y);
+void pEVSL_ParvecsGemtvWithWspace(double alp, pevsl_Parvecs *A, int nvecs,
+                                  pevsl_Parvec *x, double bet, double *y,
+                                  double *w);
+void pEVSL_ParvecsGemtv(double alp, pevsl_Parvecs *A, int nvecs,
+                        pevsl_Parvec *x, double bet, double *y);
+
+/* ---- spmat.c (INC/pevsl.h:108-116): host sparse utilities --------------- */
+void pEVSL_CsrResize(int nrow, int ncol, int nnz, pevsl_Csr *csr);
+void pEVSL_FreeCsr(pevsl_Csr *csr);
+void pEVSL_CooResize(int nrow, int ncol, int nnz, pevsl_Coo *coo);
+void pEVSL_FreeCoo(pevsl_Coo *coo);
+int pEVSL_CooToCsr(int cooidx, pevsl_Coo *coo, pevsl_Csr *csr);
+int pEVSL_CsrToCoo(pevsl_Csr *csr, int cooidx, pevsl_Coo *coo);
+int pEVSL_MatvecGen(double alp, pevsl_Csr *A, double *x, double bet,
+                    double *y);
+int pEVSL_Matvec(pevsl_Csr *A, double *x, double *y);
+void pEVSL_SortRow(pevsl_Csr *A);
+
+/* ---- utils.c / vector.c (INC/pevsl.h:119-130) --------------------------- */
+void pEVSL_SortInt(int *x, int n);
+void pEVSL_SortDouble(int n, double *v, int *ind);
+void pEVSL_LinSpace(double a, double b, int num, double *arr);
+void pEVSL_Part1d(int len, int pnum, int *idx, int *j1, int *j2, int job);
+int pEVSL_BinarySearch(int *x, int n, int key);
+void pEVSL_Vecset(int n, double t, double *v);
+double pEVSL_Wtime(void);
+void linspace(double a, double b, int num, double *arr);
+void vecset(int n, double t, double *v);
+void sort_double(int n, double *v, int *ind);
+
+/* ---- spslicer.c (INC/pevsl.h:133) ----------------------------------------
+ * Equal-count slices sli[0..n_int] (caller-allocated) from the KPM moments;
+ * the DOS integral at the npts grid points is evaluated on the GPU.
+ * Returns 0, or 1 / 2 / 3 as SRC/spslicer.c:127-136.  SRC/spslicer.c:62 */
+int pEVSL_SpslicerKpm(double *sli, double *mu, int Mdeg, double *intv,
+                      int n_int, int npts);
+
+/* ---- stats.c (INC/pevsl.h:136-137) --------------------------------------- */
+void pEVSL_StatsReset(pevsl_Data *pevsl);
+void pEVSL_StatsPrint(pevsl_Data *pevsl, FILE *fstats);
+
+/* ---- cheblanNr.c (INC/pevsl.h:140-142) -----------------------------------
+ * Filtered Lanczos without restart on p((A-cc)/dd) with full
+ * reorthogonalization.  Outputs malloc'd host arrays: *lamo, *reso (maxit
+ * entries, the first *nevOut valid) and *Wo (n x nevOut Ritz vectors, or
+ * NULL when vector output is switched off with pEVSL_SetKeepVectors).
+ * Returns 0, -1 (|gam| > 1) or 2 (zero Ritz vector).  SRC/cheblanNr.c:54 */
+int pEVSL_ChebLanNr(pevsl_Data *pevsl, double *intv, int maxit, double tol,
+                    pevsl_Parvec *vinit, pevsl_Polparams *pol, int *nevOut,
+                    double **lamo, pevsl_Parvecs **Wo, double **reso,
+                    FILE *fstats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

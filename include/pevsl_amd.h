@@ -1,0 +1,97 @@
+/*
+ * pevsl_amd.h -- extensions of the MI355X library beyond the reference API.
+ *
+ * None of these entry points has a counterpart in INC/pevsl.h; each says what
+ * it adds.  All are optional: a driver written against pevsl.h alone gets the
+ * reference behaviour (FP64 filter, libc rand() stream, one slice per call).
+ */
+#ifndef PEVSL_AMD_H
+#define PEVSL_AMD_H
+
+#include <stddef.h>
+#include <stdio.h>
+
+#include "pevsl.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Diagnostics of one filtered-Lanczos solve (one slice). */
+typedef struct {
+  int its;          /* Lanczos steps taken                                  */
+  int kdim;         /* dimension of T at the final eigen-solve              */
+  int nbreak;       /* lucky breakdowns (fresh filtered random vectors)     */
+  int nconv;        /* Ritz values of p(A) >= bar at the last test           */
+  int nev;          /* accepted eigenpairs                                  */
+  int deg;          /* filter degree                                        */
+  long long nmatvec;/* products with A (filter + Ritz residuals)            */
+  double tr;        /* restricted trace at the last test                    */
+  double t_total;   /* wall seconds of the solve                            */
+} pevsl_LanInfo;
+
+/* Accumulated device time (seconds, HIP events) by phase, and algorithmic
+ * bytes moved by the filter kernels (reset by pEVSL_StatsReset). */
+typedef struct {
+  double t_filter;        /* Chebyshev filter SpMV/SpMM steps          */
+  double t_reorth;        /* Lanczos updates + CGS-DGKS                 */
+  double t_eig;           /* tridiagonal eigen-solves                   */
+  double t_ritz;          /* Ritz vectors, residuals                    */
+  double t_kpm;           /* KPM moments                                */
+  double filter_bytes;    /* algorithmic bytes of the filter steps      */
+  double reorth_bytes;    /* algorithmic bytes of the reorthogonalization */
+  long long filter_steps; /* column-degree steps                        */
+  long long filter_launches;
+} pevsl_KernelStats;
+
+/* Filter precision: FP64 (reference) or FP32 vectors and matrix values with
+ * FP32 accumulation.  Lanczos coefficients, reorthogonalization, the
+ * tridiagonal eigen-solve and Ritz residuals stay FP64 either way. */
+#define PEVSL_PREC_FP64 0
+#define PEVSL_PREC_FP32 1
+int pEVSL_SetFilterPrecision(pevsl_Data *pevsl, int prec);
+
+/* Random vectors: PEVSL_RAND_LIBC draws (rand()-RAND_MAX/2)/(RAND_MAX/2) from
+ * libc on the host in the reference order (SRC/parvec.c:65-72);
+ * PEVSL_RAND_RADEMACHER / PEVSL_RAND_UNIFORM draw +-1 / uniform[-1,1] from a
+ * counter-based generator on the device (seeded). */
+#define PEVSL_RAND_LIBC 0
+#define PEVSL_RAND_RADEMACHER 1
+#define PEVSL_RAND_UNIFORM 2
+int pEVSL_SetRandMode(pevsl_Data *pevsl, int mode, unsigned long long seed);
+
+/* Whether ChebLanNr copies the Ritz vectors back (*Wo); 1 by default. */
+int pEVSL_SetKeepVectors(pevsl_Data *pevsl, int keep);
+
+/* SpMV lanes per row (1..64, power of two; 0 = by mean row length) and the
+ * block width of batched solves (1..64, power of two; 0 = number of slices
+ * rounded up). */
+int pEVSL_SetSpmvTuning(pevsl_Data *pevsl, int lanes_per_row, int block_width);
+
+/* Batched filtered Lanczos: nslices slices [intvs[4*s], intvs[4*s+1]] (with
+ * bounds intvs[4*s+2..3]) advance together, one column of a block SpMM each,
+ * with their own filters pols[s] (from FindPol), maxits[s], Lanczos state,
+ * breakdown handling and stopping test.  Outputs per slice as ChebLanNr:
+ * nevOut[s], lamo[s], reso[s] (malloc'd) and Wo[s] (NULL when vectors are
+ * not kept).  A one-slice call is pEVSL_ChebLanNr. */
+int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs,
+                         int *maxits, double tol, pevsl_Parvec *vinit,
+                         pevsl_Polparams *pols, int *nevOut, double **lamo,
+                         pevsl_Parvecs **Wo, double **reso, FILE *fstats);
+
+/* Diagnostics of slice s of the last ChebLanNr / ChebLanNrBatch call. */
+int pEVSL_GetLanInfo(pevsl_Data *pevsl, int s, pevsl_LanInfo *info);
+int pEVSL_GetKernelStats(pevsl_Data *pevsl, pevsl_KernelStats *st);
+
+/* y = A x with device pointers (HBM), on the instance's stream; blocks
+ * until done. */
+int pEVSL_DeviceMatvec(pevsl_Data *pevsl, const double *d_x, double *d_y);
+
+/* Device name and HBM size of the current device; 0 on success. */
+int pEVSL_DeviceInfo(char *name, int len, size_t *hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

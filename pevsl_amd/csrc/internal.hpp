@@ -1,0 +1,137 @@
+// internal.hpp -- host-side state shared by the C-ABI implementation files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+extern "C" {
+#include "pevsl.h"
+#include "pevsl_amd.h"
+}
+
+namespace pevsl {
+
+[[noreturn]] void fatal(const char *what, const char *file, int line);
+
+#define PV_HIP(call)                                                     \
+  do {                                                                   \
+    hipError_t e__ = (call);                                             \
+    if (e__ != hipSuccess) ::pevsl::fatal(hipGetErrorString(e__),        \
+                                          __FILE__, __LINE__);           \
+  } while (0)
+
+#define PV_BLAS(call)                                                    \
+  do {                                                                   \
+    rocblas_status s__ = (call);                                         \
+    if (s__ != rocblas_status_success)                                   \
+      ::pevsl::fatal(rocblas_status_to_string(s__), __FILE__, __LINE__); \
+  } while (0)
+
+// Owned device allocation.
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // grows to at least cnt elements; contents are not preserved
+  T *ensure(size_t cnt) {
+    if (cnt <= n && p) return p;
+    release();
+    PV_HIP(hipMalloc(&p, sizeof(T) * (cnt ? cnt : 1)));
+    n = cnt;
+    return p;
+  }
+};
+
+// Device copy of a CSR matrix (attached to pevsl_Parcsr.dev).
+struct DevMatrix {
+  DevCsr csr;
+  DBuf<int> ia, ja;
+  DBuf<double> a64;
+  DBuf<float> a32;
+};
+
+// Per-instance device context (attached to pevsl_Data.dev).
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  const DevCsr *A = nullptr;       // set by SetAParcsr
+  MVFunc host_mv = nullptr;        // set by SetAMatvec (host callback)
+  void *host_mv_data = nullptr;
+  int filter_prec = PEVSL_PREC_FP64;
+  int rand_mode = PEVSL_RAND_LIBC;
+  unsigned long long rand_seed = 0;
+  unsigned long long rand_offset = 0;
+  int lanes = 0;                   // lanes per row of the single-column SpMV
+  int block_width = 0;             // columns per block SpMM (0 = auto)
+  int keep_vectors = 1;
+  int chunk_cols = 512;            // Lanczos basis allocation granularity
+  std::vector<pevsl_LanInfo> info; // per slice of the last (batched) solve
+  pevsl_KernelStats kstats{};      // accumulated device timings
+  // staging for host callbacks / small copies
+  DBuf<double> cb_x, cb_y;
+  std::vector<double> h_x, h_y;
+  // scratch
+  DBuf<double> scratch64;
+};
+
+Ctx &ctx_of(pevsl_Data *p);
+const DevCsr *dev_matrix_of(pevsl_Parcsr *A);
+
+// lanes per row for a matrix with mean row length r (power of two, <= 64)
+int auto_lanes(double mean_row);
+
+// Wall-clock + device-event timing helper: measures the device time of the
+// work enqueued between start() and stop() on one stream.
+struct EvTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+  EvTimer() {
+    PV_HIP(hipEventCreate(&a));
+    PV_HIP(hipEventCreate(&b));
+  }
+  ~EvTimer() {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+  void start(hipStream_t s) { PV_HIP(hipEventRecord(a, s)); }
+  void stop(hipStream_t s) { PV_HIP(hipEventRecord(b, s)); }
+  double seconds() {
+    PV_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    PV_HIP(hipEventElapsedTime(&ms, a, b));
+    return 1e-3 * ms;
+  }
+};
+
+double wtime();
+
+// y = A x on device through the installed operator (device CSR or the host
+// callback); used by the unfused paths.
+void apply_A(Ctx &c, const double *dx, double *dy);
+
+// Random vectors: libc rand() stream on the host (reference parity,
+// SRC/parvec.c:65-72) or the device counter generator.
+void rand_vectors(Ctx &c, int n, int nvec, double *d_out, size_t ld);
+
+// Chebyshev filter y = p((A-cc)/dd) v on the device for one FP64 column
+// (v, y in HBM).  Work vectors come from the context.
+void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
+                     double *dy);
+
+}  // namespace pevsl

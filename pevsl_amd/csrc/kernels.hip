@@ -1,0 +1,988 @@
+// kernels.hip -- hand-written HIP kernels for gfx950 (CDNA4, wave64) behind
+// the pEVSL C API.
+//
+// Hot path (HBM-bound, ~1 flop/byte):
+//   * cheb_step_vec / cheb_step_blk : one degree of the Chebyshev filter
+//     (SRC/chebpol.c:538-568) fused into the sparse product: the SpMV/SpMM,
+//     the three-term recurrence and the coefficient accumulation are one pass
+//     over the matrix and the vectors.  _vec handles one column with L lanes
+//     per row; _blk handles B interleaved columns (row-major n x B) with one
+//     lane per (row, column) so that the column loads of X are contiguous.
+//   * kpm_step_blk : one degree of the KPM recurrence (SRC/kpmdos.c:129-156)
+//     for all random vectors at once, fused with the per-column dot product
+//     against v0 (block partial sums, deterministic second stage).
+//   * lan_* : Lanczos vector updates and classical Gram-Schmidt with the DGKS
+//     test (SRC/cheblanNr.c:219-246, SRC/miscla.c:211-240) as batched
+//     tall-skinny GEMV-T / GEMV-N over chunked Lanczos bases.
+// Support: Sturm-bisection restricted trace of the Lanczos tridiagonal
+// matrix, the closed-form DOS integral of the slicer, random fills.
+//
+// Every reduction is two-stage with a fixed order, so results are
+// deterministic for a given launch shape.  The filter kernels disable FMA
+// contraction so that, with one lane per row, each element follows the same
+// rounding sequence as the reference's scalar loops.
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cmath>
+#include <algorithm>
+#include <type_traits>
+
+namespace pevsl {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum over the lanes of a group of L consecutive lanes.
+template <int L>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, L);
+  return v;
+}
+template <int L>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, L);
+  return v;
+}
+
+inline int grid_for(size_t work, int threads = kThreads) {
+  return (int)((work + threads - 1) / threads);
+}
+
+// ---------------------------------------------------------------------------
+// Sparse products
+// ---------------------------------------------------------------------------
+
+// y = A x, L lanes per row (rows of a wave: 64/L).
+template <int L>
+__global__ void __launch_bounds__(kThreads)
+k_spmv_f64(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+           const double *__restrict__ a, const double *__restrict__ x,
+           double *__restrict__ y, size_t ld, int ncol) {
+#pragma clang fp contract(off)
+  const int col = blockIdx.y;
+  if (col >= ncol) return;
+  x += (size_t)col * ld;
+  y += (size_t)col * ld;
+  const size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  const int row = (int)(g / L);
+  const int lane = (int)(g % L);
+  double acc = 0.0;
+  if (row < n) {
+    const int e = ia[row + 1];
+    for (int j = ia[row] + lane; j < e; j += L) acc += a[j] * x[ja[j]];
+  }
+  if (L > 1) acc = group_sum<L>(acc);
+  if (row < n && lane == 0) y[row] = acc;
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(kThreads)
+k_cheb_vec(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+           const T *__restrict__ a, const T *__restrict__ vk,
+           const T *__restrict__ vkm1, T *__restrict__ vkp1,
+           T *__restrict__ y, T cc, T t, T mu0, T muk, int first) {
+#pragma clang fp contract(off)
+  const size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  const int row = (int)(g / L);
+  const int lane = (int)(g % L);
+  T acc = 0;
+  if (row < n) {
+    const int e = ia[row + 1];
+    for (int j = ia[row] + lane; j < e; j += L) acc += a[j] * vk[ja[j]];
+  }
+  if (L > 1) acc = group_sum<L>(acc);
+  if (row < n && lane == 0) {
+    const T v = vk[row];
+    T w = acc + (-cc) * v;  // A vk - cc vk
+    w = w * t;
+    if (first) {
+      vkp1[row] = w;
+      y[row] = v * mu0 + muk * w;
+    } else {
+      w = w + (T)(-1) * vkm1[row];
+      vkp1[row] = w;
+      y[row] = y[row] + muk * w;
+    }
+  }
+}
+
+// B interleaved columns; lane = (row_in_block, column).
+template <typename T, int B>
+__global__ void __launch_bounds__(kThreads)
+k_cheb_blk(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+           const T *__restrict__ a, const T *__restrict__ vk,
+           const T *__restrict__ vkm1, T *__restrict__ vkp1,
+           T *__restrict__ y, T cc, T t, const double *__restrict__ mus,
+           int k, int nact) {
+#pragma clang fp contract(off)
+  const size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  const int row = (int)(g / B);
+  const int c = (int)(g % B);
+  if (row >= n || c >= nact) return;
+  T acc = 0;
+  const int e = ia[row + 1];
+  for (int j = ia[row]; j < e; j++) acc += a[j] * vk[(size_t)ja[j] * B + c];
+  const size_t idx = (size_t)row * B + c;
+  const T v = vk[idx];
+  T w = acc + (-cc) * v;
+  w = w * t;
+  const T muk = (T)mus[(size_t)k * B + c];
+  if (k == 1) {
+    vkp1[idx] = w;
+    y[idx] = v * (T)mus[c] + muk * w;
+  } else {
+    w = w + (T)(-1) * vkm1[idx];
+    vkp1[idx] = w;
+    y[idx] = y[idx] + muk * w;
+  }
+}
+
+// KPM step + per-column partial dot with v0 over the block's rows.
+template <typename T, int B>
+__global__ void __launch_bounds__(kThreads)
+k_kpm_blk(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+          const T *__restrict__ a, const T *__restrict__ vk,
+          const T *__restrict__ vkm1, T *__restrict__ vkp1,
+          const T *__restrict__ v0, T ctr, T scal, double *__restrict__ part,
+          int rows_per_block) {
+#pragma clang fp contract(off)
+  constexpr int RPW = 64 / B;  // rows per wave per sweep
+  __shared__ double red[kThreads / 64][B];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane % B;
+  const int rsub = lane / B;
+  const int row0 = blockIdx.x * rows_per_block;
+  const int row_end = min(n, row0 + rows_per_block);
+  double dsum = 0.0;
+  for (int row = row0 + wave * RPW + rsub; row < row_end;
+       row += (kThreads / 64) * RPW) {
+    T acc = 0;
+    const int e = ia[row + 1];
+    for (int j = ia[row]; j < e; j++) acc += a[j] * vk[(size_t)ja[j] * B + c];
+    const size_t idx = (size_t)row * B + c;
+    T w = acc + (-ctr) * vk[idx];
+    w = w * scal;
+    w = w + (T)(-1) * vkm1[idx];
+    vkp1[idx] = w;
+    dsum += (double)w * (double)v0[idx];
+  }
+  // reduce over the rows held by this wave (lanes with equal c)
+#pragma unroll
+  for (int off = 32; off >= B; off >>= 1) dsum += __shfl_xor(dsum, off, 64);
+  if (rsub == 0) red[wave][c] = dsum;
+  __syncthreads();
+  if (threadIdx.x < B) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; w++) s += red[w][threadIdx.x];
+    part[(size_t)blockIdx.x * B + threadIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels
+// ---------------------------------------------------------------------------
+
+constexpr int kDotBlocksMax = 1024;
+
+__global__ void __launch_bounds__(kThreads)
+k_reduce_partials(const double *__restrict__ part, int nblk, int ncol,
+                  double *__restrict__ out) {
+  const int c = blockIdx.x * kThreads + threadIdx.x;
+  if (c >= ncol) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; b++) s += part[(size_t)b * ncol + c];
+  out[c] = s;
+}
+
+// Block-level sum of v into part[blockIdx.x] (all threads participate).
+__device__ __forceinline__ void block_sum_store(double v, double *dst) {
+  __shared__ double red[kThreads / 64];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kThreads / 64; w++) s += red[w];
+    *dst = s;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_dot_part(int n, const double *__restrict__ x, const double *__restrict__ y,
+           double *__restrict__ part) {
+  double acc = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < (size_t)n;
+       i += (size_t)gridDim.x * kThreads)
+    acc += x[i] * y[i];
+  block_sum_store(acc, part + blockIdx.x);
+}
+
+__global__ void k_scale(int n, double *__restrict__ x,
+                        const double *__restrict__ inv_dev, double hs) {
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= (size_t)n) return;
+  const double s = inv_dev ? (1.0 / *inv_dev) : hs;
+  x[i] = x[i] * s;
+}
+
+__global__ void k_axpy(int n, double a, const double *__restrict__ x,
+                       double *__restrict__ y) {
+#pragma clang fp contract(off)
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < (size_t)n) y[i] = y[i] + a * x[i];
+}
+
+template <typename A, typename B>
+__global__ void k_convert(size_t n, const A *__restrict__ x, B *__restrict__ y) {
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) y[i] = (B)x[i];
+}
+
+__global__ void k_zero(size_t n, double *__restrict__ x) {
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+template <typename T>
+__global__ void k_pack(int n, int B, int ncol, const double *const *cols,
+                       T *__restrict__ blk) {
+  const size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  const size_t row = g / B;
+  const int c = (int)(g % B);
+  if (row >= (size_t)n) return;
+  blk[g] = c < ncol ? (T)cols[c][row] : (T)0;
+}
+
+template <typename T>
+__global__ void k_unpack(int n, int B, int ncol, const T *__restrict__ blk,
+                         double *const *cols) {
+  const size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  const size_t row = g / B;
+  const int c = (int)(g % B);
+  if (row >= (size_t)n || c >= ncol) return;
+  cols[c][row] = (double)blk[g];
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos (batched over slices; blockIdx.y = slice)
+// ---------------------------------------------------------------------------
+
+struct LanArgs {
+  int n;
+  size_t ld;
+  int chunk_shift;
+  int maxchunks;
+  double **vtab;
+  double *alpha, *beta, *nrm2a, *nrm2b;
+  int *pass2, *active;
+  double *h;
+  int hcap;
+  double *part;
+};
+
+LanArgs make_args(const LanBatch &L) {
+  LanArgs a;
+  a.n = L.n;
+  a.ld = L.ld;
+  a.chunk_shift = L.chunk_shift;
+  a.maxchunks = L.maxchunks;
+  a.vtab = L.vtab;
+  a.alpha = L.alpha;
+  a.beta = L.beta;
+  a.nrm2a = L.nrm2a;
+  a.nrm2b = L.nrm2b;
+  a.pass2 = L.pass2;
+  a.active = L.active;
+  a.h = L.h;
+  a.hcap = L.hcap;
+  a.part = L.part;
+  return a;
+}
+
+__device__ __forceinline__ double *lan_col(const LanArgs &L, int s, int j) {
+  const int mask = (1 << L.chunk_shift) - 1;
+  return L.vtab[(size_t)s * L.maxchunks + (j >> L.chunk_shift)] +
+         (size_t)(j & mask) * L.ld;
+}
+
+// number of row blocks used by the per-slice vector reductions
+inline int lan_vec_blocks(int n) {
+  int b = (n + kThreads * 4 - 1) / (kThreads * 4);
+  return b < 1 ? 1 : (b > kDotBlocksMax ? kDotBlocksMax : b);
+}
+
+// znew -= beta zold (k > 0); partial <z, znew>
+__global__ void __launch_bounds__(kThreads)
+k_lan_alpha(LanArgs L, int k) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.y;
+  if (!L.active[s]) return;
+  double *znew = lan_col(L, s, k + 1);
+  const double *z = lan_col(L, s, k);
+  const double *zold = k > 0 ? lan_col(L, s, k - 1) : nullptr;
+  const double mb = -L.beta[s];
+  double acc = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < (size_t)L.n;
+       i += (size_t)gridDim.x * kThreads) {
+    double w = znew[i];
+    if (zold) {
+      w = w + mb * zold[i];
+      znew[i] = w;
+    }
+    acc += z[i] * w;
+  }
+  block_sum_store(acc, L.part + (size_t)s * gridDim.x + blockIdx.x);
+}
+
+// znew -= alpha z; partial |znew|^2
+__global__ void __launch_bounds__(kThreads)
+k_lan_sub_alpha(LanArgs L, int k) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.y;
+  if (!L.active[s]) return;
+  double *znew = lan_col(L, s, k + 1);
+  const double *z = lan_col(L, s, k);
+  const double ma = -L.alpha[s];
+  double acc = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < (size_t)L.n;
+       i += (size_t)gridDim.x * kThreads) {
+    const double w = znew[i] + ma * z[i];
+    znew[i] = w;
+    acc += w * w;
+  }
+  block_sum_store(acc, L.part + (size_t)s * gridDim.x + blockIdx.x);
+}
+
+// per-slice sums of the vector partials into one of the scalar arrays
+// which: 0 alpha, 1 nrm2a, 2 nrm2b (pass 0, sets pass2), 3 nrm2b (pass 1)
+__global__ void k_lan_reduce(LanArgs L, int nblk, int which) {
+  const int s = blockIdx.x;
+  if (!L.active[s]) return;
+  if (which == 3 && !L.pass2[s]) return;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 64)
+    v += L.part[(size_t)s * nblk + b];
+  v = wave_sum(v);
+  if (threadIdx.x != 0) return;
+  if (which == 0) {
+    L.alpha[s] = v;
+  } else if (which == 1) {
+    L.nrm2a[s] = v;
+  } else {
+    L.nrm2b[s] = v;
+    if (which == 2) {
+      const double eta = 1.0 / sqrt(2.0);
+      L.pass2[s] = (sqrt(v) > eta * sqrt(L.nrm2a[s])) ? 0 : 1;
+    }
+  }
+}
+
+// GEMV-T partials: part[(s*hcap + j)*nrb + rb] = sum over the row block of
+// V_s(:,j) * znew_s, for j in this block's column chunk.
+constexpr int kGtRpt = 8;            // rows per thread
+constexpr int kGtRows = kThreads * kGtRpt;
+constexpr int kGtCols = 64;          // columns per block
+
+__global__ void __launch_bounds__(kThreads)
+k_lan_gemvt(LanArgs L, int k, int pass, int nrb) {
+  const int s = blockIdx.z;
+  if (!L.active[s]) return;
+  if (pass && !L.pass2[s]) return;
+  const int ncols = k + 1;
+  const int j0 = blockIdx.y * kGtCols;
+  if (j0 >= ncols) return;
+  const int jn = min(kGtCols, ncols - j0);
+  const int rb = blockIdx.x;
+  const double *z = lan_col(L, s, k + 1);
+  const int row0 = rb * kGtRows + threadIdx.x;
+  double zr[kGtRpt];
+#pragma unroll
+  for (int i = 0; i < kGtRpt; i++) {
+    const int r = row0 + i * kThreads;
+    zr[i] = r < L.n ? z[r] : 0.0;
+  }
+  __shared__ double red[kThreads / 64][kGtCols];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int jj = 0;
+  for (; jj + 4 <= jn; jj += 4) {
+    const double *v0 = lan_col(L, s, j0 + jj);
+    const double *v1 = lan_col(L, s, j0 + jj + 1);
+    const double *v2 = lan_col(L, s, j0 + jj + 2);
+    const double *v3 = lan_col(L, s, j0 + jj + 3);
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+    for (int i = 0; i < kGtRpt; i++) {
+      const int r = row0 + i * kThreads;
+      if (r < L.n) {
+        a0 += v0[r] * zr[i];
+        a1 += v1[r] * zr[i];
+        a2 += v2[r] * zr[i];
+        a3 += v3[r] * zr[i];
+      }
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    a3 = wave_sum(a3);
+    if (lane == 0) {
+      red[wave][jj] = a0;
+      red[wave][jj + 1] = a1;
+      red[wave][jj + 2] = a2;
+      red[wave][jj + 3] = a3;
+    }
+  }
+  for (; jj < jn; jj++) {
+    const double *v0 = lan_col(L, s, j0 + jj);
+    double a0 = 0;
+#pragma unroll
+    for (int i = 0; i < kGtRpt; i++) {
+      const int r = row0 + i * kThreads;
+      if (r < L.n) a0 += v0[r] * zr[i];
+    }
+    a0 = wave_sum(a0);
+    if (lane == 0) red[wave][jj] = a0;
+  }
+  __syncthreads();
+  if (threadIdx.x < jn) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; w++) t += red[w][threadIdx.x];
+    L.part[((size_t)s * L.hcap + j0 + threadIdx.x) * nrb + rb] = t;
+  }
+}
+
+// h_s[j] = sum_rb part[(s*hcap + j)*nrb + rb]
+__global__ void __launch_bounds__(kThreads)
+k_lan_reduce_h(LanArgs L, int k, int pass, int nrb) {
+  const int s = blockIdx.y;
+  if (!L.active[s]) return;
+  if (pass && !L.pass2[s]) return;
+  const int j = blockIdx.x * kThreads + threadIdx.x;
+  if (j > k) return;
+  const double *p = L.part + ((size_t)s * L.hcap + j) * nrb;
+  double v = 0.0;
+  for (int b = 0; b < nrb; b++) v += p[b];
+  L.h[(size_t)s * L.hcap + j] = v;
+}
+
+// znew -= V(:,0:k) h, column by column (the order of DGEMV 'N'), with the
+// block partial of |znew|^2.
+constexpr int kGnRpt = 4;
+__global__ void __launch_bounds__(kThreads)
+k_lan_gemvn(LanArgs L, int k, int pass, int nblk) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.y;
+  if (!L.active[s]) return;
+  if (pass && !L.pass2[s]) return;
+  double *z = lan_col(L, s, k + 1);
+  const double *h = L.h + (size_t)s * L.hcap;
+  double acc[kGnRpt];
+  int rows[kGnRpt];
+#pragma unroll
+  for (int i = 0; i < kGnRpt; i++) {
+    rows[i] = (blockIdx.x * kGnRpt + i) * kThreads + threadIdx.x;
+    acc[i] = rows[i] < L.n ? z[rows[i]] : 0.0;
+  }
+  for (int j = 0; j <= k; j++) {
+    const double *v = lan_col(L, s, j);
+    const double mh = -h[j];
+#pragma unroll
+    for (int i = 0; i < kGnRpt; i++)
+      if (rows[i] < L.n) acc[i] = acc[i] + mh * v[rows[i]];
+  }
+  double sq = 0.0;
+#pragma unroll
+  for (int i = 0; i < kGnRpt; i++) {
+    if (rows[i] < L.n) {
+      z[rows[i]] = acc[i];
+      sq += acc[i] * acc[i];
+    }
+  }
+  block_sum_store(sq, L.part + (size_t)s * nblk + blockIdx.x);
+}
+
+__global__ void k_lan_scale(LanArgs L, int k) {
+  const int s = blockIdx.y;
+  if (!L.active[s]) return;
+  double *z = lan_col(L, s, k + 1);
+  const double t = 1.0 / L.beta[s];
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < (size_t)L.n;
+       i += (size_t)gridDim.x * kThreads)
+    z[i] = z[i] * t;
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric tridiagonal: Sturm counts and bisection (LAPACK DLAEBZ-style
+// pivot guard, W. Kahan's method)
+// ---------------------------------------------------------------------------
+
+__device__ int sturm_less(int m, const double *__restrict__ d,
+                          const double *__restrict__ e, double x,
+                          double pivmin) {
+  int cnt = 0;
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  if (q <= 0.0) cnt++;
+  for (int i = 1; i < m; i++) {
+    q = d[i] - e[i - 1] * e[i - 1] / q - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q <= 0.0) cnt++;
+  }
+  return cnt;
+}
+
+// ws: [0] lower bound, [1] upper bound, [2] pivmin, [3] first index >= thr
+__global__ void k_tri_prep(int m, const double *__restrict__ d,
+                           const double *__restrict__ e, double thr,
+                           double *__restrict__ ws) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double lo = d[0], hi = d[0], emax2 = 0.0;
+  for (int i = 0; i < m; i++) {
+    double r = 0.0;
+    if (i > 0) r += fabs(e[i - 1]);
+    if (i < m - 1) r += fabs(e[i]);
+    lo = fmin(lo, d[i] - r);
+    hi = fmax(hi, d[i] + r);
+    if (i < m - 1) emax2 = fmax(emax2, e[i] * e[i]);
+  }
+  const double pivmin = DBL_MIN * fmax(1.0, emax2);
+  const double span = fmax(fabs(lo), fabs(hi));
+  lo -= 2.0 * DBL_EPSILON * span + pivmin;
+  hi += 2.0 * DBL_EPSILON * span + pivmin;
+  ws[0] = lo;
+  ws[1] = hi;
+  ws[2] = pivmin;
+  ws[3] = (thr <= lo) ? 0.0 : (double)sturm_less(m, d, e, thr, pivmin);
+}
+
+// lam[idx] for idx >= first (first = ws[3], or 0 when all == 1)
+__global__ void __launch_bounds__(kThreads)
+k_tri_bisect(int m, const double *__restrict__ d, const double *__restrict__ e,
+             double thr, const double *__restrict__ ws,
+             double *__restrict__ lam, int all) {
+  const int idx = blockIdx.x * kThreads + threadIdx.x;
+  if (idx >= m) return;
+  const int first = all ? 0 : (int)ws[3];
+  if (idx < first) return;
+  double lo = (all || thr <= ws[0]) ? ws[0] : thr, hi = ws[1];
+  const double pivmin = ws[2];
+  for (int it = 0; it < 128; it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin ||
+        mid <= lo || mid >= hi)
+      break;
+    if (sturm_less(m, d, e, mid, pivmin) > idx)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  lam[idx] = 0.5 * (lo + hi);
+}
+
+__global__ void k_tri_sum(int m, const double *__restrict__ ws,
+                          const double *__restrict__ lam,
+                          double *__restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int first = (int)ws[3];
+  double s = 0.0;
+  for (int i = first; i < m; i++) s += lam[i];
+  out[0] = s;
+  out[1] = (double)(m - first);
+}
+
+// ---------------------------------------------------------------------------
+// Misc
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kThreads)
+k_intchx(int Mdeg, const double *__restrict__ mu, int npts,
+         const double *__restrict__ th, double th0, double *__restrict__ yi) {
+#pragma clang fp contract(off)
+  const int j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= npts) return;
+  const double t = th[j];
+  double y = mu[0] * (th0 - t);
+  for (int k = 1; k <= Mdeg; k++) {
+    const double val0 = sin(k * th0) / k;
+    y += mu[k] * (val0 - sin(k * t) / k);
+  }
+  yi[j] = y;
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_col_dots(int n, const double *__restrict__ X, size_t ldx,
+           const double *__restrict__ Y, size_t ldy, double *__restrict__ part) {
+  const int c = blockIdx.y;
+  const double *x = X + (size_t)c * ldx;
+  const double *y = Y + (size_t)c * ldy;
+  double acc = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < (size_t)n;
+       i += (size_t)gridDim.x * kThreads)
+    acc += x[i] * y[i];
+  block_sum_store(acc, part + (size_t)c * gridDim.x + blockIdx.x);
+}
+
+__global__ void k_col_reduce(int ncol, int nblk, const double *__restrict__ part,
+                             double *__restrict__ out) {
+  const int c = blockIdx.x;
+  if (c >= ncol) return;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 64) v += part[(size_t)c * nblk + b];
+  v = wave_sum(v);
+  if (threadIdx.x == 0) out[c] = v;
+}
+
+__global__ void k_col_scale(int n, double *__restrict__ X, size_t ld,
+                            const double *__restrict__ f) {
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= (size_t)n) return;
+  X[(size_t)blockIdx.y * ld + i] *= f[blockIdx.y];
+}
+
+__global__ void k_col_axpy(int n, const double *__restrict__ a,
+                           const double *__restrict__ X, size_t ldx,
+                           double *__restrict__ Y, size_t ldy) {
+#pragma clang fp contract(off)
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= (size_t)n) return;
+  const int c = blockIdx.y;
+  Y[(size_t)c * ldy + i] = Y[(size_t)c * ldy + i] + a[c] * X[(size_t)c * ldx + i];
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__global__ void k_rand(size_t n, double *__restrict__ x, uint64_t seed,
+                       uint64_t offset, int rademacher) {
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t r = splitmix64(seed ^ splitmix64(offset + i));
+  if (rademacher)
+    x[i] = (r >> 63) ? 1.0 : -1.0;
+  else
+    x[i] = (double)(r >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Launchers
+// ===========================================================================
+
+void spmv_cols_f64(const DevCsr &A, const double *X, double *Y, int ncol,
+                   size_t ld, int lanes, hipStream_t s) {
+  if (A.n == 0 || ncol == 0) return;
+  dim3 grid(grid_for((size_t)A.n * lanes), ncol);
+  switch (lanes) {
+#define CASE(L)                                                              \
+  case L:                                                                    \
+    hipLaunchKernelGGL(k_spmv_f64<L>, grid, dim3(kThreads), 0, s, A.n, A.ia, \
+                       A.ja, A.a64, X, Y, ld, ncol);                         \
+    break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
+#undef CASE
+    default:
+      hipLaunchKernelGGL(k_spmv_f64<1>, grid, dim3(kThreads), 0, s, A.n, A.ia,
+                         A.ja, A.a64, X, Y, ld, ncol);
+  }
+}
+
+void spmv_f64(const DevCsr &A, const double *x, double *y, int lanes,
+              hipStream_t s) {
+  spmv_cols_f64(A, x, y, 1, (size_t)A.n, lanes, s);
+}
+
+template <typename T>
+void cheb_step_vec(const DevCsr &A, const T *vk, const T *vkm1, T *vkp1, T *y,
+                   double cc, double t, double mu0, double muk, bool first,
+                   int lanes, hipStream_t s) {
+  if (A.n == 0) return;
+  const T *a = std::is_same<T, double>::value ? (const T *)A.a64
+                                              : (const T *)A.a32;
+  dim3 grid(grid_for((size_t)A.n * lanes));
+  switch (lanes) {
+#define CASE(L)                                                                \
+  case L:                                                                      \
+    hipLaunchKernelGGL((k_cheb_vec<T, L>), grid, dim3(kThreads), 0, s, A.n,    \
+                       A.ia, A.ja, a, vk, vkm1, vkp1, y, (T)cc, (T)t, (T)mu0,  \
+                       (T)muk, first ? 1 : 0);                                 \
+    break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
+#undef CASE
+    default:
+      hipLaunchKernelGGL((k_cheb_vec<T, 1>), grid, dim3(kThreads), 0, s, A.n,
+                         A.ia, A.ja, a, vk, vkm1, vkp1, y, (T)cc, (T)t, (T)mu0,
+                         (T)muk, first ? 1 : 0);
+  }
+}
+template void cheb_step_vec<double>(const DevCsr &, const double *,
+                                    const double *, double *, double *, double,
+                                    double, double, double, bool, int,
+                                    hipStream_t);
+template void cheb_step_vec<float>(const DevCsr &, const float *, const float *,
+                                   float *, float *, double, double, double,
+                                   double, bool, int, hipStream_t);
+
+template <typename T>
+void cheb_step_blk(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
+                   T *y, double cc, double t, const double *mus, int k,
+                   int nact, hipStream_t s) {
+  if (A.n == 0 || nact <= 0) return;
+  const T *a = std::is_same<T, double>::value ? (const T *)A.a64
+                                              : (const T *)A.a32;
+  dim3 grid(grid_for((size_t)A.n * B));
+  switch (B) {
+#define CASE(BB)                                                              \
+  case BB:                                                                    \
+    hipLaunchKernelGGL((k_cheb_blk<T, BB>), grid, dim3(kThreads), 0, s, A.n,  \
+                       A.ia, A.ja, a, vk, vkm1, vkp1, y, (T)cc, (T)t, mus, k, \
+                       nact);                                                 \
+    break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
+#undef CASE
+    default:
+      break;
+  }
+}
+template void cheb_step_blk<double>(const DevCsr &, int, const double *,
+                                    const double *, double *, double *, double,
+                                    double, const double *, int, int,
+                                    hipStream_t);
+template void cheb_step_blk<float>(const DevCsr &, int, const float *,
+                                   const float *, float *, float *, double,
+                                   double, const double *, int, int,
+                                   hipStream_t);
+
+// rows per block of the KPM kernel: enough rows that every block does a few
+// sweeps, and at most ~4096 blocks.
+static int kpm_rows_per_block(int n, int B) {
+  const int rpw = 64 / B;
+  const int sweep = (kThreads / 64) * rpw;
+  int rpb = sweep * 4;
+  while ((n + rpb - 1) / rpb > 4096) rpb *= 2;
+  return rpb;
+}
+
+int kpm_part_blocks(int n, int B) {
+  const int rpb = kpm_rows_per_block(n, B);
+  return (n + rpb - 1) / rpb;
+}
+
+template <typename T>
+int kpm_step_blk(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
+                 const T *v0, double ctr, double scal, double *part,
+                 hipStream_t s) {
+  const int rpb = kpm_rows_per_block(A.n, B);
+  const int nb = (A.n + rpb - 1) / rpb;
+  const T *a = std::is_same<T, double>::value ? (const T *)A.a64
+                                              : (const T *)A.a32;
+  switch (B) {
+#define CASE(BB)                                                             \
+  case BB:                                                                   \
+    hipLaunchKernelGGL((k_kpm_blk<T, BB>), dim3(nb), dim3(kThreads), 0, s,   \
+                       A.n, A.ia, A.ja, a, vk, vkm1, vkp1, v0, (T)ctr,       \
+                       (T)scal, part, rpb);                                  \
+    break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
+#undef CASE
+    default:
+      return 0;
+  }
+  return nb;
+}
+template int kpm_step_blk<double>(const DevCsr &, int, const double *,
+                                  const double *, double *, const double *,
+                                  double, double, double *, hipStream_t);
+template int kpm_step_blk<float>(const DevCsr &, int, const float *,
+                                 const float *, float *, const float *, double,
+                                 double, double *, hipStream_t);
+
+void reduce_partials(const double *part, int nblk, int ncol, double *out,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(grid_for(ncol)), dim3(kThreads),
+                     0, s, part, nblk, ncol, out);
+}
+
+int dot_part_blocks(int n) { return lan_vec_blocks(n); }
+
+int dot_partials(int n, const double *x, const double *y, double *part,
+                 hipStream_t s) {
+  const int nb = lan_vec_blocks(n);
+  hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(kThreads), 0, s, n, x, y,
+                     part);
+  return nb;
+}
+
+void scale_vec(int n, double *x, const double *inv_scale_dev, double hostscale,
+               hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x,
+                     inv_scale_dev, hostscale);
+}
+
+void axpy_vec(int n, double a, const double *x, double *y, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(kThreads), 0, s, n, a, x,
+                     y);
+}
+
+void copy_f64_to_f32(size_t n, const double *x, float *y, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)),
+                     dim3(kThreads), 0, s, n, x, y);
+}
+void copy_f32_to_f64(size_t n, const float *x, double *y, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)),
+                     dim3(kThreads), 0, s, n, x, y);
+}
+void fill_zero_f64(size_t n, double *x, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_zero, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x);
+}
+
+template <typename T>
+void pack_cols(int n, int B, int ncol, const double *const *cols, T *blk,
+               hipStream_t s) {
+  hipLaunchKernelGGL((k_pack<T>), dim3(grid_for((size_t)n * B)),
+                     dim3(kThreads), 0, s, n, B, ncol, cols, blk);
+}
+template <typename T>
+void unpack_cols(int n, int B, int ncol, const T *blk, double *const *cols,
+                 hipStream_t s) {
+  hipLaunchKernelGGL((k_unpack<T>), dim3(grid_for((size_t)n * B)),
+                     dim3(kThreads), 0, s, n, B, ncol, blk, cols);
+}
+template void pack_cols<double>(int, int, int, const double *const *, double *,
+                                hipStream_t);
+template void pack_cols<float>(int, int, int, const double *const *, float *,
+                               hipStream_t);
+template void unpack_cols<double>(int, int, int, const double *,
+                                  double *const *, hipStream_t);
+template void unpack_cols<float>(int, int, int, const float *, double *const *,
+                                 hipStream_t);
+
+size_t lan_part_doubles(int n, int nslice, int hcap) {
+  const size_t nrb = (size_t)(n + kGtRows - 1) / kGtRows;
+  const size_t nbn = (size_t)(n + kThreads * kGnRpt - 1) / (kThreads * kGnRpt);
+  const size_t per = std::max(std::max((size_t)hcap * nrb, nbn),
+                              (size_t)lan_vec_blocks(n));
+  return (size_t)nslice * per + 64;
+}
+
+void lan_alpha(const LanBatch &L, int k, hipStream_t s) {
+  const int nb = lan_vec_blocks(L.n);
+  LanArgs a = make_args(L);
+  hipLaunchKernelGGL(k_lan_alpha, dim3(nb, L.nslice), dim3(kThreads), 0, s, a,
+                     k);
+  hipLaunchKernelGGL(k_lan_reduce, dim3(L.nslice), dim3(64), 0, s, a, nb, 0);
+}
+
+void lan_subtract_alpha(const LanBatch &L, int k, hipStream_t s) {
+  const int nb = lan_vec_blocks(L.n);
+  LanArgs a = make_args(L);
+  hipLaunchKernelGGL(k_lan_sub_alpha, dim3(nb, L.nslice), dim3(kThreads), 0, s,
+                     a, k);
+  hipLaunchKernelGGL(k_lan_reduce, dim3(L.nslice), dim3(64), 0, s, a, nb, 1);
+}
+
+void lan_cgs_pass(const LanBatch &L, int k, int pass, hipStream_t s) {
+  LanArgs a = make_args(L);
+  const int nrb = (L.n + kGtRows - 1) / kGtRows;
+  const int ncb = (k + 1 + kGtCols - 1) / kGtCols;
+  hipLaunchKernelGGL(k_lan_gemvt, dim3(nrb, ncb, L.nslice), dim3(kThreads), 0,
+                     s, a, k, pass, nrb);
+  hipLaunchKernelGGL(k_lan_reduce_h, dim3(grid_for(k + 1), L.nslice),
+                     dim3(kThreads), 0, s, a, k, pass, nrb);
+  const int nbn = (L.n + kThreads * kGnRpt - 1) / (kThreads * kGnRpt);
+  hipLaunchKernelGGL(k_lan_gemvn, dim3(nbn, L.nslice), dim3(kThreads), 0, s, a,
+                     k, pass, nbn);
+  hipLaunchKernelGGL(k_lan_reduce, dim3(L.nslice), dim3(64), 0, s, a, nbn,
+                     pass ? 3 : 2);
+}
+
+void lan_scale_new(const LanBatch &L, int k, hipStream_t s) {
+  LanArgs a = make_args(L);
+  const int nb = lan_vec_blocks(L.n);
+  hipLaunchKernelGGL(k_lan_scale, dim3(nb, L.nslice), dim3(kThreads), 0, s, a,
+                     k);
+}
+
+void tridiag_trace_above(int m, const double *d, const double *e, double thr,
+                         double *lam, double *out, hipStream_t s) {
+  // ws lives in out[2..5]
+  double *ws = out + 2;
+  hipLaunchKernelGGL(k_tri_prep, dim3(1), dim3(64), 0, s, m, d, e, thr, ws);
+  hipLaunchKernelGGL(k_tri_bisect, dim3(grid_for(m)), dim3(kThreads), 0, s, m,
+                     d, e, thr, (const double *)ws, lam, 0);
+  hipLaunchKernelGGL(k_tri_sum, dim3(1), dim3(64), 0, s, m,
+                     (const double *)ws, (const double *)lam, out);
+}
+
+void tridiag_eigvals(int m, const double *d, const double *e, double *lam,
+                     hipStream_t s) {
+  // uses lam[m..m+3] as workspace for the bounds: caller gives m+4 doubles
+  double *ws = lam + m;
+  hipLaunchKernelGGL(k_tri_prep, dim3(1), dim3(64), 0, s, m, d, e, -HUGE_VAL,
+                     ws);
+  hipLaunchKernelGGL(k_tri_bisect, dim3(grid_for(m)), dim3(kThreads), 0, s, m,
+                     d, e, -HUGE_VAL, (const double *)ws, lam, 1);
+}
+
+void kpm_intchx(int Mdeg, const double *mu, int npts, const double *th,
+                double th0, double *yi, hipStream_t s) {
+  hipLaunchKernelGGL(k_intchx, dim3(grid_for(npts)), dim3(kThreads), 0, s,
+                     Mdeg, mu, npts, th, th0, yi);
+}
+
+int col_part_blocks(int n) { return lan_vec_blocks(n); }
+
+void col_dots(int n, int ncol, const double *X, size_t ldx, const double *Y,
+              size_t ldy, double *part, double *out, hipStream_t s) {
+  if (ncol <= 0) return;
+  const int nb = lan_vec_blocks(n);
+  hipLaunchKernelGGL(k_col_dots, dim3(nb, ncol), dim3(kThreads), 0, s, n, X,
+                     ldx, Y, ldy, part);
+  hipLaunchKernelGGL(k_col_reduce, dim3(ncol), dim3(64), 0, s, ncol, nb,
+                     (const double *)part, out);
+}
+
+void col_scale(int n, int ncol, double *X, size_t ld, const double *f,
+               hipStream_t s) {
+  if (ncol <= 0 || n == 0) return;
+  hipLaunchKernelGGL(k_col_scale, dim3(grid_for(n), ncol), dim3(kThreads), 0,
+                     s, n, X, ld, f);
+}
+
+void col_axpy(int n, int ncol, const double *a, const double *X, size_t ldx,
+              double *Y, size_t ldy, hipStream_t s) {
+  if (ncol <= 0 || n == 0) return;
+  hipLaunchKernelGGL(k_col_axpy, dim3(grid_for(n), ncol), dim3(kThreads), 0,
+                     s, n, a, X, ldx, Y, ldy);
+}
+
+void rand_fill(size_t n, double *x, uint64_t seed, uint64_t offset,
+               int rademacher, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_rand, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x,
+                     seed, offset, rademacher);
+}
+
+}  // namespace pevsl

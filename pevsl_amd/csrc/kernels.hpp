@@ -1,0 +1,151 @@
+// kernels.hpp -- launchers of the HIP kernels (kernels.hip) used by the host
+// side of the library.  All launchers enqueue on the given stream and never
+// synchronize; shapes are validated by the callers (see check_* in dev.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace pevsl {
+
+// Device CSR: int32 row pointers / column indices, values in FP64 and FP32.
+struct DevCsr {
+  int n = 0;       // rows == cols
+  int nnz = 0;
+  int *ia = nullptr;
+  int *ja = nullptr;
+  double *a64 = nullptr;
+  float *a32 = nullptr;
+  int max_row = 0;  // longest row
+};
+
+// ---- sparse products -------------------------------------------------------
+// y = A x, one column.  lanes = lanes per row in {1,2,4,8,16,32,64}.
+void spmv_f64(const DevCsr &A, const double *x, double *y, int lanes,
+              hipStream_t s);
+// Y(:,c) = A X(:,c) for ncol column-major columns with leading dim ld.
+void spmv_cols_f64(const DevCsr &A, const double *X, double *Y, int ncol,
+                   size_t ld, int lanes, hipStream_t s);
+
+// One Chebyshev degree step of the filter for one column (T = float/double):
+//   first:  vkp1 = t (A vk - cc vk);        y = mu0 vk + muk vkp1
+//   else:   vkp1 = t (A vk - cc vk) - vkm1; y = y + muk vkp1
+template <typename T>
+void cheb_step_vec(const DevCsr &A, const T *vk, const T *vkm1, T *vkp1, T *y,
+                   double cc, double t, double mu0, double muk, bool first,
+                   int lanes, hipStream_t s);
+
+// Same for B interleaved columns (row-major n x B block, column fastest).
+// mus: device array [(deg_max+1) x B] of per-column coefficients; k = degree
+// index of this step (1..); columns c >= nact are left untouched.
+template <typename T>
+void cheb_step_blk(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
+                   T *y, double cc, double t, const double *mus, int k,
+                   int nact, hipStream_t s);
+
+// KPM step on B interleaved columns:
+//   vkp1 = scal (A vk - ctr vk) - vkm1,   part[blk][c] = sum_rows vkp1*v0
+// returns the number of row blocks (partials per column) written.
+template <typename T>
+int kpm_step_blk(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
+                 const T *v0, double ctr, double scal, double *part,
+                 hipStream_t s);
+int kpm_part_blocks(int n, int B);
+
+// ---- dense / vector kernels (FP64 unless stated) --------------------------
+// out[c] = sum_b part[b*ncol + c], b < nblk (fixed order); out may be strided.
+void reduce_partials(const double *part, int nblk, int ncol, double *out,
+                     hipStream_t s);
+// part[blk] = sum x_i*y_i over the block's rows; returns nblk
+int dot_partials(int n, const double *x, const double *y, double *part,
+                 hipStream_t s);
+int dot_part_blocks(int n);
+void scale_vec(int n, double *x, const double *inv_scale_dev, double hostscale,
+               hipStream_t s);
+void axpy_vec(int n, double a, const double *x, double *y, hipStream_t s);
+void copy_f64_to_f32(size_t n, const double *x, float *y, hipStream_t s);
+void copy_f32_to_f64(size_t n, const float *x, double *y, hipStream_t s);
+void fill_zero_f64(size_t n, double *x, hipStream_t s);
+
+// Pack/unpack between one column-major FP64 vector per slice and an
+// interleaved n x B block of type T.  cols[c] points to the n-vector of
+// column c (device array of pointers), c < ncol.
+template <typename T>
+void pack_cols(int n, int B, int ncol, const double *const *cols, T *blk,
+               hipStream_t s);
+template <typename T>
+void unpack_cols(int n, int B, int ncol, const T *blk, double *const *cols,
+                 hipStream_t s);
+
+// ---- Lanczos on a batch of slices ----------------------------------------
+// Slices are independent; slice s owns V_s (chunked columns, leading dim ld)
+// and per-step scalars.  Column j of slice s is
+//   vtab[s*maxchunks + (j >> chunk_shift)] + (size_t)(j & chunk_mask)*ld
+struct LanBatch {
+  int n = 0;
+  size_t ld = 0;
+  int nslice = 0;
+  int chunk_shift = 9;
+  int maxchunks = 0;
+  double **vtab = nullptr;  // device [nslice*maxchunks]
+  // device scalars per slice
+  double *alpha = nullptr;  // [nslice]
+  double *beta = nullptr;   // [nslice] (beta of the previous step)
+  double *nrm2a = nullptr;  // [nslice] |v|^2 before reorth pass
+  double *nrm2b = nullptr;  // [nslice] |v|^2 after a reorth pass
+  int *pass2 = nullptr;     // [nslice] 1 if the 2nd DGKS pass is needed
+  int *active = nullptr;    // [nslice] 1 if the slice takes part
+  double *h = nullptr;      // [nslice * hcap]
+  int hcap = 0;
+  double *part = nullptr;   // partial sums workspace
+  size_t part_cap = 0;      // doubles
+};
+
+// doubles of LanBatch::part needed for n rows, nslice slices and hcap
+// reorthogonalization coefficients per slice
+size_t lan_part_doubles(int n, int nslice, int hcap);
+
+// znew_s -= beta_s zold_s (k > 0);  alpha_s = <z_s, znew_s>
+void lan_alpha(const LanBatch &L, int k, hipStream_t s);
+// znew_s -= alpha_s z_s;  nrm2a_s = |znew_s|^2
+void lan_subtract_alpha(const LanBatch &L, int k, hipStream_t s);
+// One classical Gram-Schmidt pass against V_s(:,0:k): h = V^T znew,
+// znew -= V h, nrm2b = |znew|^2.  pass = 0 always runs; pass = 1 runs only
+// for slices whose pass2 flag is set (decided on the device after pass 0).
+void lan_cgs_pass(const LanBatch &L, int k, int pass, hipStream_t s);
+// znew_s /= beta_s for active slices (beta read from the device array).
+void lan_scale_new(const LanBatch &L, int k, hipStream_t s);
+
+// ---- tridiagonal (FP64) ---------------------------------------------------
+// Restricted trace of T (diag d[0..m-1], sub-diag e[0..m-2]): eigenvalues
+// >= thr by Sturm bisection.  out[0] = sum, out[1] = count (as double).
+// Workspace: lam (m doubles).
+void tridiag_trace_above(int m, const double *d, const double *e, double thr,
+                         double *lam, double *out, hipStream_t s);
+// All eigenvalues of T by bisection (ascending), for tests and small m.
+void tridiag_eigvals(int m, const double *d, const double *e, double *lam,
+                     hipStream_t s);
+
+// ---- column-major multi-vectors (Ritz stage) ------------------------------
+// part must hold ncol * col_part_blocks(n) doubles.
+int col_part_blocks(int n);
+// out[c] = <X(:,c), Y(:,c)>
+void col_dots(int n, int ncol, const double *X, size_t ldx, const double *Y,
+              size_t ldy, double *part, double *out, hipStream_t s);
+// X(:,c) *= f[c]
+void col_scale(int n, int ncol, double *X, size_t ld, const double *f,
+               hipStream_t s);
+// Y(:,c) += a[c] X(:,c)
+void col_axpy(int n, int ncol, const double *a, const double *X, size_t ldx,
+              double *Y, size_t ldy, hipStream_t s);
+
+// ---- misc -----------------------------------------------------------------
+// y[j] = integral of the KPM DOS at x[j] (closed form, per-point sequential
+// sum over degrees as the reference does).
+void kpm_intchx(int Mdeg, const double *mu, int npts, const double *th,
+                double th0, double *yi, hipStream_t s);
+// Rademacher (+-1) or uniform [-1,1] counter-based random numbers.
+void rand_fill(size_t n, double *x, uint64_t seed, uint64_t offset,
+               int rademacher, hipStream_t s);
+
+}  // namespace pevsl

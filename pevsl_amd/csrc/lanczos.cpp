@@ -1,0 +1,777 @@
+// lanczos.cpp -- filtered Lanczos without restart (pEVSL_ChebLanNr) and its
+// batched form (pEVSL_ChebLanNrBatch) on the GPU.
+//
+// Per slice the control flow is SRC/cheblanNr.c:54-448: filter the initial
+// vector, then for k = 0.. : z_{k+1} = p(A) z_k, three-term update with
+// alpha/beta, full reorthogonalization by classical Gram-Schmidt with the
+// DGKS test (SRC/miscla.c:211-240), lucky-breakdown restart from a fresh
+// filtered random vector, and every `cycle` steps from `Ntest` on the
+// restricted trace of the Ritz values of p(A) above `bar` as the stopping
+// test; at the end the Ritz pairs of the selected Ritz values are formed and
+// accepted if their Rayleigh quotient lies in the slice and the residual
+// norm is below tol.
+//
+// GPU mapping:
+//   * all slices of a batch step in lockstep; their current Lanczos vectors
+//     are the columns of one interleaved block, filtered together by a block
+//     SpMM with per-column coefficients and degrees (kernels.hip
+//     k_cheb_blk); a single slice uses the one-column kernel (k_cheb_vec);
+//   * the Lanczos bases live in HBM in chunks of `chunk_cols` columns that
+//     are allocated as the basis grows, so memory follows the actual step
+//     count rather than maxit;
+//   * alpha/beta, the DGKS decision and the norms stay on the device; the
+//     host reads two scalars per slice per step for the breakdown test and
+//     the tridiagonal matrix;
+//   * the restricted trace uses GPU Sturm bisection on T_k; the final
+//     eigenvectors of T come from rocSOLVER (divide and conquer), the Ritz
+//     vectors from rocBLAS DGEMM over the basis chunks, the residuals from the
+//     SpMV and column kernels.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include <rocsolver/rocsolver.h>
+
+#include "internal.hpp"
+
+namespace pevsl {
+
+namespace {
+
+constexpr int kNtest = 30;   // first convergence test (SRC/cheblanNr.c:82)
+constexpr int kCycle = 20;   // test period (SRC/cheblanNr.c:84)
+
+struct Slice {
+  int idx = 0;
+  double aa = 0, bb = 0, bar = 0, gam = 0;
+  int maxit = 0;
+  const pevsl_Polparams *pol = nullptr;
+  std::vector<double *> chunks;
+  std::vector<double> dT, eT;
+  double wn = 0.0;
+  int nwn = 0;
+  double tr0 = 0.0, tr1 = 0.0;
+  int kdim = 0, nconv = 0, nbreak = 0, its = 0;
+  bool done = false;
+  long long nmatvec = 0;
+  double t_start = 0.0;
+  // results
+  int rc = 0;
+  std::vector<double> lam, res;
+  std::vector<double> vecs;  // accepted Ritz vectors (host, n x nev)
+};
+
+size_t filter_launch_bytes(const DevCsr &A, int elem, int ncol, bool first) {
+  const double vec = (double)A.n * elem;
+  return (size_t)((double)A.nnz * (elem + 4) + 4.0 * (A.n + 1) +
+                  (first ? 3.0 : 5.0) * vec * ncol);
+}
+
+}  // namespace
+
+// y = p((A - cc)/dd) v for one FP64 column in HBM (SRC/chebpol.c:509-574).
+void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
+                     double *dy) {
+  const int n = c.A ? c.A->n : (int)c.h_x.size();
+  const int m = pol->deg;
+  const double *mu = pol->mu;
+  const double cc = pol->cc, dd = pol->dd;
+  const hipStream_t s = c.stream;
+  if (m == 0 || !c.A) {
+    // operator given as a host callback (or trivial degree): unfused steps
+    double *w = c.scratch64.ensure((size_t)4 * n);
+    double *vk = w, *vkp1 = w + n, *vkm1 = w + 2 * (size_t)n;
+    PV_HIP(hipMemcpyAsync(vk, dv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    PV_HIP(hipMemcpyAsync(dy, vk, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    scale_vec(n, dy, nullptr, mu[0], s);
+    for (int k = 1; k <= m; k++) {
+      const double t = k == 1 ? 1.0 / dd : 2.0 / dd;
+      apply_A(c, vk, vkp1);
+      axpy_vec(n, -cc, vk, vkp1, s);
+      scale_vec(n, vkp1, nullptr, t, s);
+      if (k > 1) axpy_vec(n, -1.0, vkm1, vkp1, s);
+      axpy_vec(n, mu[k], vkp1, dy, s);
+      std::swap(vkm1, vk);
+      std::swap(vk, vkp1);
+    }
+    c.kstats.filter_steps += m;
+    return;
+  }
+  const DevCsr &A = *c.A;
+  const int lanes = c.lanes ? c.lanes : auto_lanes((double)A.nnz / std::max(1, n));
+  if (c.filter_prec == PEVSL_PREC_FP64) {
+    double *w = c.scratch64.ensure((size_t)3 * n);
+    double *b[3] = {w, w + n, w + 2 * (size_t)n};
+    const double *vk = dv, *vkm1 = nullptr;
+    int nb = 0;
+    for (int k = 1; k <= m; k++) {
+      double *vkp1 = b[nb];
+      nb = (nb + 1) % 3;
+      cheb_step_vec<double>(A, vk, vkm1, vkp1, dy, cc,
+                            k == 1 ? 1.0 / dd : 2.0 / dd, mu[0], mu[k], k == 1,
+                            lanes, s);
+      vkm1 = vk;
+      vk = vkp1;
+      c.kstats.filter_bytes += (double)filter_launch_bytes(A, 8, 1, k == 1);
+    }
+  } else {
+    float *w = reinterpret_cast<float *>(c.scratch64.ensure((size_t)5 * n / 2 + 8));
+    float *x32 = w, *y32 = w + n;
+    float *b[3] = {w + 2 * (size_t)n, w + 3 * (size_t)n, w + 4 * (size_t)n};
+    copy_f64_to_f32((size_t)n, dv, x32, s);
+    const float *vk = x32, *vkm1 = nullptr;
+    int nb = 0;
+    for (int k = 1; k <= m; k++) {
+      float *vkp1 = b[nb];
+      nb = (nb + 1) % 3;
+      cheb_step_vec<float>(A, vk, vkm1, vkp1, y32, cc,
+                           k == 1 ? 1.0 / dd : 2.0 / dd, mu[0], mu[k], k == 1,
+                           lanes, s);
+      vkm1 = vk;
+      vk = vkp1;
+      c.kstats.filter_bytes += (double)filter_launch_bytes(A, 4, 1, k == 1);
+    }
+    copy_f32_to_f64((size_t)n, y32, dy, s);
+  }
+  c.kstats.filter_steps += m;
+  c.kstats.filter_launches += m;
+}
+
+namespace {
+
+class BatchSolver {
+ public:
+  BatchSolver(Ctx &c, pevsl_Data *p, int ns) : c_(c), p_(p), ns_(ns), S_(ns) {}
+
+  int run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit,
+          pevsl_Polparams *pols, FILE *fstats);
+  void outputs(int *nevOut, double **lamo, pevsl_Parvecs **Wo, double **reso,
+               pevsl_Parvec *vinit);
+
+ private:
+  double *col(int s, int j) {
+    const int sh = L_.chunk_shift;
+    return S_[(size_t)s].chunks[(size_t)(j >> sh)] + (size_t)(j & ((1 << sh) - 1)) * L_.ld;
+  }
+  void ensure_column(int s, int j);
+  void sync_vtab();
+  void set_active(const std::vector<int> &act);
+  void filter_step(const std::vector<int> &act, int k);
+  void filter_block(const std::vector<int> &grp, int k);
+  void reorth_and_norm(const std::vector<int> &act, int k);
+  void breakdown(int s, int k);
+  void convergence_test(int s, int k);
+  void ritz(int s, double tol);
+  void free_basis(int s);
+
+  Ctx &c_;
+  pevsl_Data *p_;
+  int ns_;
+  int n_ = 0;
+  std::vector<Slice> S_;
+  LanBatch L_;
+  DBuf<double *> vtab_;
+  std::vector<double *> vtab_h_;
+  DBuf<double> scal_;  // alpha, beta, nrm2a, nrm2b
+  DBuf<int> flags_;    // pass2, active
+  DBuf<double> h_, part_;
+  DBuf<double> blk_;   // filter blocks
+  DBuf<double> mus_;
+  DBuf<double *> colptr_;
+  DBuf<double> tri_;   // tridiagonal work
+  double *pinned_ = nullptr;
+  int *pinned_i_ = nullptr;
+  std::vector<int> mus_key_;
+};
+
+void BatchSolver::ensure_column(int s, int j) {
+  Slice &sl = S_[(size_t)s];
+  const size_t need = (size_t)(j >> L_.chunk_shift) + 1;
+  bool grew = false;
+  while (sl.chunks.size() < need) {
+    double *p = nullptr;
+    PV_HIP(hipMalloc(&p, sizeof(double) * L_.ld * ((size_t)1 << L_.chunk_shift)));
+    sl.chunks.push_back(p);
+    vtab_h_[(size_t)s * L_.maxchunks + sl.chunks.size() - 1] = p;
+    grew = true;
+  }
+  if (grew) sync_vtab();
+}
+
+void BatchSolver::sync_vtab() {
+  PV_HIP(hipMemcpyAsync(vtab_.p, vtab_h_.data(), sizeof(double *) * vtab_h_.size(),
+                        hipMemcpyHostToDevice, c_.stream));
+  PV_HIP(hipStreamSynchronize(c_.stream));
+}
+
+void BatchSolver::set_active(const std::vector<int> &act) {
+  for (int s = 0; s < ns_; s++) pinned_i_[s] = 0;
+  for (int s : act) pinned_i_[s] = 1;
+  PV_HIP(hipMemcpyAsync(L_.active, pinned_i_, sizeof(int) * ns_,
+                        hipMemcpyHostToDevice, c_.stream));
+}
+
+void BatchSolver::free_basis(int s) {
+  PV_HIP(hipStreamSynchronize(c_.stream));
+  for (size_t i = 0; i < S_[(size_t)s].chunks.size(); i++) {
+    PV_HIP(hipFree(S_[(size_t)s].chunks[i]));
+    vtab_h_[(size_t)s * L_.maxchunks + i] = nullptr;
+  }
+  S_[(size_t)s].chunks.clear();
+}
+
+// Block filter for a group of at most B slices (sorted by degree, highest
+// first): z_{k+1} = p_s(A) z_k for every slice s of the group.
+void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
+  const DevCsr &A = *c_.A;
+  const int g = (int)grp.size();
+  int B = c_.block_width;
+  if (B == 0 || B < g) {
+    B = 1;
+    while (B < g) B *= 2;
+  }
+  const bool f32 = c_.filter_prec == PEVSL_PREC_FP32;
+  const size_t elem = f32 ? 4 : 8;
+  const size_t blk_elems = (size_t)n_ * B;
+  char *base = reinterpret_cast<char *>(blk_.ensure((4 * blk_elems * elem) / 8 + 8));
+  void *xb[4];
+  for (int i = 0; i < 4; i++) xb[i] = base + (size_t)i * blk_elems * elem;
+  // per-column coefficients [(degmax+1) x B]
+  int degmax = 0;
+  std::vector<int> key;
+  for (int s : grp) {
+    degmax = std::max(degmax, S_[(size_t)s].pol->deg);
+    key.push_back(s);
+  }
+  key.push_back(B);
+  if (key != mus_key_) {
+    std::vector<double> mh((size_t)(degmax + 1) * B, 0.0);
+    for (int ci = 0; ci < g; ci++) {
+      const pevsl_Polparams *pol = S_[(size_t)grp[(size_t)ci]].pol;
+      for (int j = 0; j <= pol->deg; j++) mh[(size_t)j * B + ci] = pol->mu[j];
+    }
+    mus_.ensure(mh.size());
+    PV_HIP(hipMemcpyAsync(mus_.p, mh.data(), sizeof(double) * mh.size(),
+                          hipMemcpyHostToDevice, c_.stream));
+    mus_key_ = key;
+  }
+  // column pointers: inputs z_k, outputs z_{k+1}
+  std::vector<double *> cp((size_t)2 * B, nullptr);
+  for (int ci = 0; ci < g; ci++) {
+    cp[(size_t)ci] = col(grp[(size_t)ci], k);
+    cp[(size_t)B + ci] = col(grp[(size_t)ci], k + 1);
+  }
+  colptr_.ensure(cp.size());
+  PV_HIP(hipMemcpyAsync(colptr_.p, cp.data(), sizeof(double *) * cp.size(),
+                        hipMemcpyHostToDevice, c_.stream));
+  const double cc = S_[(size_t)grp[0]].pol->cc, dd = S_[(size_t)grp[0]].pol->dd;
+  std::vector<int> degs;
+  for (int s : grp) degs.push_back(S_[(size_t)s].pol->deg);
+  auto run = [&](auto *tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    T *x = (T *)xb[0], *y = (T *)xb[1];
+    T *b[3] = {(T *)xb[2], (T *)xb[3], nullptr};
+    pack_cols<T>(n_, B, g, colptr_.p, x, c_.stream);
+    // third work buffer: reuse x once it is no longer read (after step 2)
+    const T *vk = x, *vkm1 = nullptr;
+    T *bufs[3] = {b[0], b[1], x};
+    int nb = 0;
+    for (int d = 1; d <= degmax; d++) {
+      int nact = 0;
+      while (nact < g && degs[(size_t)nact] >= d) nact++;
+      T *vkp1 = bufs[nb];
+      // never overwrite the buffers read in this step
+      if ((const T *)vkp1 == vk || (const T *)vkp1 == vkm1) {
+        nb = (nb + 1) % 3;
+        vkp1 = bufs[nb];
+      }
+      nb = (nb + 1) % 3;
+      cheb_step_blk<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd,
+                       mus_.p, d, nact, c_.stream);
+      c_.kstats.filter_bytes += (double)filter_launch_bytes(A, (int)elem, nact, d == 1);
+      c_.kstats.filter_steps += nact;
+      c_.kstats.filter_launches++;
+      vkm1 = vk;
+      vk = vkp1;
+    }
+    unpack_cols<T>(n_, B, g, y, colptr_.p + B, c_.stream);
+  };
+  if (f32)
+    run((float *)nullptr);
+  else
+    run((double *)nullptr);
+}
+
+void BatchSolver::filter_step(const std::vector<int> &act, int k) {
+  for (int s : act) ensure_column(s, k + 1);
+  const bool single = act.size() == 1 || !c_.A || c_.block_width == 1;
+  if (single) {
+    for (int s : act) {
+      cheb_filter_col(c_, S_[(size_t)s].pol, col(s, k), col(s, k + 1));
+      S_[(size_t)s].nmatvec += S_[(size_t)s].pol->deg;
+    }
+    return;
+  }
+  // groups of at most 64 slices, each sorted by degree (descending)
+  std::vector<int> order(act);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return S_[(size_t)a].pol->deg > S_[(size_t)b].pol->deg;
+  });
+  const int gmax = c_.block_width ? c_.block_width : 64;
+  for (size_t i = 0; i < order.size(); i += (size_t)gmax) {
+    std::vector<int> grp(order.begin() + (long)i,
+                         order.begin() + (long)std::min(order.size(), i + (size_t)gmax));
+    if (grp.size() == 1) {
+      cheb_filter_col(c_, S_[(size_t)grp[0]].pol, col(grp[0], k), col(grp[0], k + 1));
+    } else {
+      filter_block(grp, k);
+    }
+    for (int s : grp) S_[(size_t)s].nmatvec += S_[(size_t)s].pol->deg;
+  }
+}
+
+void BatchSolver::reorth_and_norm(const std::vector<int> &act, int k) {
+  set_active(act);
+  lan_alpha(L_, k, c_.stream);
+  lan_subtract_alpha(L_, k, c_.stream);
+  lan_cgs_pass(L_, k, 0, c_.stream);
+  lan_cgs_pass(L_, k, 1, c_.stream);
+  const double vecb = 8.0 * n_;
+  c_.kstats.reorth_bytes += act.size() * (4 * vecb + 3 * vecb + 2.0 * (k + 1) * vecb + 3 * vecb);
+}
+
+// Lucky breakdown (SRC/cheblanNr.c:248-281): replace z_{k+1} by a filtered
+// random vector orthogonalized against the basis; beta = 0.
+void BatchSolver::breakdown(int s, int k) {
+  Slice &sl = S_[(size_t)s];
+  sl.nbreak++;
+  double *vr = c_.cb_x.ensure((size_t)n_);
+  rand_vectors(c_, n_, 1, vr, (size_t)n_);
+  cheb_filter_col(c_, sl.pol, vr, col(s, k + 1));
+  sl.nmatvec += sl.pol->deg;
+  std::vector<int> one{s};
+  set_active(one);
+  // nrm2a of the fresh vector, then the DGKS passes
+  PV_HIP(hipMemsetAsync(L_.alpha + s, 0, sizeof(double), c_.stream));
+  lan_subtract_alpha(L_, k, c_.stream);  // alpha = 0: only |v|^2
+  lan_cgs_pass(L_, k, 0, c_.stream);
+  lan_cgs_pass(L_, k, 1, c_.stream);
+  PV_HIP(hipMemcpyAsync(pinned_, L_.nrm2b + s, sizeof(double), hipMemcpyDeviceToHost,
+                        c_.stream));
+  PV_HIP(hipStreamSynchronize(c_.stream));
+  const double beta = sqrt(pinned_[0]);
+  PV_HIP(hipMemcpyAsync(L_.beta + s, &beta, sizeof(double), hipMemcpyHostToDevice,
+                        c_.stream));
+  lan_scale_new(L_, k, c_.stream);
+  const double zero = 0.0;
+  PV_HIP(hipMemcpyAsync(L_.beta + s, &zero, sizeof(double), hipMemcpyHostToDevice,
+                        c_.stream));
+  PV_HIP(hipStreamSynchronize(c_.stream));
+}
+
+void BatchSolver::convergence_test(int s, int k) {
+  Slice &sl = S_[(size_t)s];
+  const int kdim = k + 1;
+  double *d = tri_.ensure((size_t)3 * kdim + 16);
+  double *e = d + kdim, *lam = e + kdim, *out = lam + kdim;
+  PV_HIP(hipMemcpyAsync(d, sl.dT.data(), sizeof(double) * kdim, hipMemcpyHostToDevice,
+                        c_.stream));
+  if (kdim > 1)
+    PV_HIP(hipMemcpyAsync(e, sl.eT.data(), sizeof(double) * (kdim - 1),
+                          hipMemcpyHostToDevice, c_.stream));
+  const double t0 = wtime();
+  tridiag_trace_above(kdim, d, e, sl.bar - 5 * DBL_EPSILON, lam, out, c_.stream);
+  PV_HIP(hipMemcpyAsync(pinned_, out, sizeof(double) * 2, hipMemcpyDeviceToHost,
+                        c_.stream));
+  PV_HIP(hipStreamSynchronize(c_.stream));
+  p_->stats->t_eig += wtime() - t0;
+  sl.kdim = kdim;
+  sl.tr1 = pinned_[0];
+  sl.nconv = (int)pinned_[1];
+  if (fabs(sl.tr1 - sl.tr0) < 2e-12 || fabs(sl.tr1) + fabs(sl.tr0) < 1e-10) {
+    sl.done = true;
+  }
+  sl.tr0 = sl.tr1;
+  if (k == sl.maxit - 1) sl.done = true;
+}
+
+// Ritz pairs of slice s and their acceptance (SRC/cheblanNr.c:339-413).
+void BatchSolver::ritz(int s, double tol) {
+  Slice &sl = S_[(size_t)s];
+  const hipStream_t st = c_.stream;
+  const int kdim = sl.kdim;
+  const double t0 = wtime();
+  // eigenpairs of T_kdim (ascending), divide and conquer on the device
+  DBuf<double> D, E, C;
+  DBuf<int> info;
+  D.ensure((size_t)kdim);
+  E.ensure((size_t)std::max(kdim, 1));
+  C.ensure((size_t)kdim * kdim);
+  info.ensure(1);
+  PV_HIP(hipMemcpyAsync(D.p, sl.dT.data(), sizeof(double) * kdim, hipMemcpyHostToDevice, st));
+  if (kdim > 1)
+    PV_HIP(hipMemcpyAsync(E.p, sl.eT.data(), sizeof(double) * (kdim - 1),
+                          hipMemcpyHostToDevice, st));
+  PV_BLAS(rocsolver_dstedc(c_.blas, rocblas_evect_tridiagonal, kdim, D.p, E.p, C.p,
+                           kdim, info.p));
+  std::vector<double> ev((size_t)kdim);
+  int hinfo = 0;
+  PV_HIP(hipMemcpyAsync(ev.data(), D.p, sizeof(double) * kdim, hipMemcpyDeviceToHost, st));
+  PV_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  PV_HIP(hipStreamSynchronize(st));
+  if (hinfo) fprintf(stderr, "pevsl_amd: STEDC info %d\n", hinfo);
+  p_->stats->t_eig += wtime() - t0;
+  int i0 = kdim;
+  while (i0 > 0 && ev[(size_t)i0 - 1] >= sl.bar) i0--;
+  const int nsel = kdim - i0;
+  const int CH = 256;  // Ritz vectors per batch
+  DBuf<double> U, W, part, red;
+  const size_t ld = L_.ld;
+  U.ensure(ld * (size_t)std::min(CH, std::max(nsel, 1)));
+  W.ensure(ld * (size_t)std::min(CH, std::max(nsel, 1)));
+  part.ensure((size_t)CH * col_part_blocks(n_) + 8);
+  red.ensure((size_t)4 * CH);
+  std::vector<double> hv((size_t)CH), fac((size_t)CH), aneg((size_t)CH), tq((size_t)CH),
+      rr((size_t)CH);
+  const int csz = 1 << L_.chunk_shift;
+  const double one = 1.0, zero = 0.0;
+  for (int cs = 0; cs < nsel && sl.rc == 0; cs += CH) {
+    const int nb = std::min(CH, nsel - cs);
+    // U = V(:, 0:kdim) * C(:, i0+cs : i0+cs+nb), accumulated over basis chunks
+    for (int j0 = 0, ci = 0; j0 < kdim; j0 += csz, ci++) {
+      const int kc = std::min(csz, kdim - j0);
+      PV_BLAS(rocblas_dgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n_, nb,
+                            kc, &one, sl.chunks[(size_t)ci], (int)ld,
+                            C.p + (size_t)(i0 + cs) * kdim + j0, kdim,
+                            ci == 0 ? &zero : &one, U.p, (int)ld));
+    }
+    col_dots(n_, nb, U.p, ld, U.p, ld, part.p, red.p, st);
+    PV_HIP(hipMemcpyAsync(hv.data(), red.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+    PV_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nb; i++) {
+      const double t = sqrt(hv[(size_t)i]);
+      if (t == 0.0) {
+        sl.rc = 2;  // zero Ritz vector (SRC/cheblanNr.c:377-379)
+        break;
+      }
+      fac[(size_t)i] = 1.0 / t;
+    }
+    if (sl.rc) break;
+    PV_HIP(hipMemcpyAsync(red.p, fac.data(), sizeof(double) * nb, hipMemcpyHostToDevice, st));
+    col_scale(n_, nb, U.p, ld, red.p, st);
+    // W = A U, Rayleigh quotients
+    if (c_.A) {
+      spmv_cols_f64(*c_.A, U.p, W.p, nb, ld,
+                    c_.lanes ? c_.lanes : auto_lanes((double)c_.A->nnz / std::max(1, n_)),
+                    st);
+    } else {
+      for (int i = 0; i < nb; i++) apply_A(c_, U.p + (size_t)i * ld, W.p + (size_t)i * ld);
+    }
+    sl.nmatvec += nb;
+    col_dots(n_, nb, W.p, ld, U.p, ld, part.p, red.p, st);
+    PV_HIP(hipMemcpyAsync(tq.data(), red.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+    PV_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nb; i++) aneg[(size_t)i] = -tq[(size_t)i];
+    PV_HIP(hipMemcpyAsync(red.p, aneg.data(), sizeof(double) * nb, hipMemcpyHostToDevice, st));
+    col_axpy(n_, nb, red.p, U.p, ld, W.p, ld, st);
+    col_dots(n_, nb, W.p, ld, W.p, ld, part.p, red.p + CH, st);
+    PV_HIP(hipMemcpyAsync(rr.data(), red.p + CH, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                          st));
+    PV_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nb; i++) {
+      const double t = tq[(size_t)i];
+      if (t < sl.aa - DBL_EPSILON || t > sl.bb + DBL_EPSILON) continue;
+      const double r = sqrt(rr[(size_t)i]);
+      if (r < tol) {
+        sl.lam.push_back(t);
+        sl.res.push_back(r);
+        if (c_.keep_vectors) {
+          const size_t off = sl.vecs.size();
+          sl.vecs.resize(off + (size_t)n_);
+          PV_HIP(hipMemcpyAsync(sl.vecs.data() + off, U.p + (size_t)i * ld,
+                                sizeof(double) * n_, hipMemcpyDeviceToHost, st));
+        }
+      }
+    }
+    PV_HIP(hipStreamSynchronize(st));
+  }
+  p_->stats->t_ritz += wtime() - t0;
+}
+
+int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit,
+                     pevsl_Polparams *pols, FILE *fstats) {
+  n_ = p_->n;
+  const hipStream_t st = c_.stream;
+  const double tall = wtime();
+  int kmax = 0;
+  for (int s = 0; s < ns_; s++) {
+    Slice &sl = S_[(size_t)s];
+    const double *iv = intvs + 4 * (size_t)s;
+    sl.idx = s;
+    sl.aa = iv[0];
+    sl.bb = iv[1];
+    sl.pol = &pols[s];
+    sl.bar = pols[s].bar;
+    sl.gam = pols[s].gam;
+    sl.maxit = std::min(p_->N, maxits[s]);
+    sl.t_start = tall;
+    // invalid interval: no eigenvalues, return 0 (SRC/cheblanNr.c:96-100)
+    if (iv[0] >= iv[1] || iv[0] >= iv[3] || iv[1] <= iv[2] || sl.maxit <= 0) {
+      sl.done = true;
+      continue;
+    }
+    if (sl.gam > 1.0 || sl.gam < -1.0) {
+      fprintf(stdout, "gamB error %.15e\n", sl.gam);
+      return -1;
+    }
+    sl.dT.assign((size_t)sl.maxit, 0.0);
+    sl.eT.assign((size_t)sl.maxit, 0.0);
+    kmax = std::max(kmax, sl.maxit);
+  }
+  // batch workspace
+  int shift = 0;
+  while ((1 << shift) < std::max(64, c_.chunk_cols)) shift++;
+  L_.n = n_;
+  L_.ld = ((size_t)n_ + 63) / 64 * 64;
+  L_.nslice = ns_;
+  L_.chunk_shift = shift;
+  L_.maxchunks = (kmax + 1 + (1 << shift) - 1) >> shift;
+  vtab_h_.assign((size_t)ns_ * L_.maxchunks, nullptr);
+  L_.vtab = vtab_.ensure(vtab_h_.size());
+  double *sc = scal_.ensure((size_t)4 * ns_);
+  L_.alpha = sc;
+  L_.beta = sc + ns_;
+  L_.nrm2a = sc + 2 * ns_;
+  L_.nrm2b = sc + 3 * ns_;
+  int *fl = flags_.ensure((size_t)2 * ns_);
+  L_.pass2 = fl;
+  L_.active = fl + ns_;
+  PV_HIP(hipMemsetAsync(sc, 0, sizeof(double) * 4 * ns_, st));
+  PV_HIP(hipMemsetAsync(fl, 0, sizeof(int) * 2 * ns_, st));
+  L_.hcap = kmax + 1;
+  L_.h = h_.ensure((size_t)ns_ * L_.hcap);
+  L_.part_cap = lan_part_doubles(n_, ns_, L_.hcap);
+  L_.part = part_.ensure(L_.part_cap);
+  PV_HIP(hipHostMalloc(&pinned_, sizeof(double) * (2 * ns_ + 8)));
+  PV_HIP(hipHostMalloc(&pinned_i_, sizeof(int) * (ns_ + 8)));
+
+  std::vector<int> act;
+  for (int s = 0; s < ns_; s++)
+    if (!S_[(size_t)s].done) act.push_back(s);
+  // V(:,0) = p(A) vinit / |.|  (FILTER_VINIT, SRC/cheblanNr.c:169-194)
+  double *dvi = c_.cb_y.ensure((size_t)n_);
+  PV_HIP(hipMemcpyAsync(dvi, vinit->data, sizeof(double) * n_, hipMemcpyHostToDevice, st));
+  EvTimer ev_f, ev_r;
+  double t_filter = 0, t_reorth = 0;
+  for (int s : act) {
+    ensure_column(s, 0);
+    cheb_filter_col(c_, S_[(size_t)s].pol, dvi, col(s, 0));
+    S_[(size_t)s].nmatvec += S_[(size_t)s].pol->deg;
+  }
+  if (!act.empty()) {
+    // |v|: reuse the Lanczos kernels with the basis column 0 as "znew"
+    // (z = zold = nothing): nrm2a via lan_subtract_alpha with alpha = 0 is
+    // not applicable at k = -1, so use the dot kernels directly.
+    DBuf<double> pt;
+    pt.ensure((size_t)dot_part_blocks(n_) + 8);
+    for (int s : act) {
+      const int nb = dot_partials(n_, col(s, 0), col(s, 0), pt.p, st);
+      reduce_partials(pt.p, nb, 1, pt.p + nb, st);
+      PV_HIP(hipMemcpyAsync(pinned_, pt.p + nb, sizeof(double), hipMemcpyDeviceToHost, st));
+      PV_HIP(hipStreamSynchronize(st));
+      scale_vec(n_, col(s, 0), nullptr, 1.0 / sqrt(pinned_[0]), st);
+    }
+  }
+  for (int k = 0; !act.empty(); k++) {
+    // ---- z_{k+1} = p(A) z_k for every active slice
+    ev_f.start(st);
+    filter_step(act, k);
+    ev_f.stop(st);
+    // ---- three-term update and full reorthogonalization
+    ev_r.start(st);
+    reorth_and_norm(act, k);
+    ev_r.stop(st);
+    for (int s = 0; s < ns_; s++) {
+      pinned_[s] = 0;
+      pinned_[ns_ + s] = 0;
+    }
+    PV_HIP(hipMemcpyAsync(pinned_, L_.alpha, sizeof(double) * ns_, hipMemcpyDeviceToHost, st));
+    PV_HIP(hipMemcpyAsync(pinned_ + ns_, L_.nrm2b, sizeof(double) * ns_,
+                          hipMemcpyDeviceToHost, st));
+    PV_HIP(hipStreamSynchronize(st));
+    t_filter += ev_f.seconds();
+    t_reorth += ev_r.seconds();
+    std::vector<int> brk;
+    std::vector<double> betas((size_t)ns_, 0.0);
+    for (int s : act) {
+      Slice &sl = S_[(size_t)s];
+      const double alpha = pinned_[s];
+      const double beta = sqrt(pinned_[ns_ + s]);
+      sl.dT[(size_t)k] = alpha;
+      sl.wn += fabs(alpha);
+      sl.wn += 2.0 * beta;
+      sl.nwn += 3;
+      betas[(size_t)s] = beta;
+      if (beta * sl.nwn < orthTol * sl.wn) brk.push_back(s);
+    }
+    // regular normalization for the slices without breakdown
+    PV_HIP(hipMemcpyAsync(L_.beta, betas.data(), sizeof(double) * ns_, hipMemcpyHostToDevice,
+                          st));
+    std::vector<int> reg;
+    for (int s : act)
+      if (std::find(brk.begin(), brk.end(), s) == brk.end()) reg.push_back(s);
+    if (!reg.empty()) {
+      set_active(reg);
+      lan_scale_new(L_, k, st);
+    }
+    for (int s : brk) {
+      if (fstats) fprintf(fstats, "it %4d: Lucky breakdown, beta = %.15e\n", k, betas[(size_t)s]);
+      breakdown(s, k);
+      betas[(size_t)s] = 0.0;
+    }
+    for (int s : act) {
+      Slice &sl = S_[(size_t)s];
+      sl.eT[(size_t)k] = betas[(size_t)s];
+      sl.its = k + 1;
+      if ((k < kNtest || (k - kNtest) % kCycle != 0) && k != sl.maxit - 1) continue;
+      convergence_test(s, k);
+      if (fstats) fprintf(fstats, "  k %4d:   nconv %4d  tr1 %21.15e\n", k, sl.nconv, sl.tr1);
+    }
+    // retire converged slices: Ritz pairs, then release their basis
+    std::vector<int> still;
+    for (int s : act) {
+      Slice &sl = S_[(size_t)s];
+      if (!sl.done) {
+        still.push_back(s);
+        continue;
+      }
+      if (k == sl.maxit - 1 && fstats)
+        fprintf(fstats, "The max number of iterations [%d] has been reached.\n", sl.maxit);
+      ritz(s, tol);
+      free_basis(s);
+    }
+    act.swap(still);
+    if (!act.empty()) {
+      // next step writes z_{k+2}; every active slice has beta set on device
+    }
+  }
+  p_->stats->t_polAv += t_filter;
+  p_->stats->t_reorth += t_reorth;
+  c_.kstats.t_filter += t_filter;
+  c_.kstats.t_reorth += t_reorth;
+  p_->stats->t_iter += wtime() - tall;
+  c_.info.assign((size_t)ns_, pevsl_LanInfo{});
+  for (int s = 0; s < ns_; s++) {
+    Slice &sl = S_[(size_t)s];
+    pevsl_LanInfo &in = c_.info[(size_t)s];
+    in.its = sl.its;
+    in.kdim = sl.kdim;
+    in.nbreak = sl.nbreak;
+    in.nconv = sl.nconv;
+    in.nev = (int)sl.lam.size();
+    in.deg = sl.pol ? sl.pol->deg : 0;
+    in.nmatvec = sl.nmatvec;
+    in.tr = sl.tr1;
+    in.t_total = wtime() - tall;
+  }
+  PV_HIP(hipHostFree(pinned_));
+  PV_HIP(hipHostFree(pinned_i_));
+  pinned_ = nullptr;
+  pinned_i_ = nullptr;
+  int rc = 0;
+  for (int s = 0; s < ns_; s++)
+    if (S_[(size_t)s].rc) rc = S_[(size_t)s].rc;
+  return rc;
+}
+
+void BatchSolver::outputs(int *nevOut, double **lamo, pevsl_Parvecs **Wo, double **reso,
+                          pevsl_Parvec *vinit) {
+  for (int s = 0; s < ns_; s++) {
+    Slice &sl = S_[(size_t)s];
+    const int nev = (int)sl.lam.size();
+    nevOut[s] = nev;
+    const size_t cap = (size_t)std::max(sl.maxit, std::max(nev, 1));
+    double *lam, *res;
+    PEVSL_MALLOC(lam, cap, double);
+    PEVSL_MALLOC(res, cap, double);
+    for (int i = 0; i < nev; i++) {
+      lam[i] = sl.lam[(size_t)i];
+      res[i] = sl.res[(size_t)i];
+    }
+    lamo[s] = lam;
+    reso[s] = res;
+    if (Wo) {
+      if (c_.keep_vectors) {
+        pevsl_Parvecs *W;
+        PEVSL_MALLOC(W, 1, pevsl_Parvecs);
+        pEVSL_ParvecsDuplParvec(vinit, std::max(nev, 1), vinit->n_local, W);
+        W->n_vecs = nev;
+        if (nev) memcpy(W->data, sl.vecs.data(), sizeof(double) * (size_t)nev * n_);
+        Wo[s] = W;
+      } else {
+        Wo[s] = NULL;
+      }
+    }
+  }
+}
+
+}  // namespace
+}  // namespace pevsl
+
+using namespace pevsl;
+
+extern "C" {
+
+int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs, int *maxits,
+                         double tol, pevsl_Parvec *vinit, pevsl_Polparams *pols,
+                         int *nevOut, double **lamo, pevsl_Parvecs **Wo, double **reso,
+                         FILE *fstats) {
+  Ctx &c = ctx_of(pevsl);
+  if (nslices <= 0) return 0;
+  if (!c.A && !c.host_mv) return -3;
+  if (pevsl->n != vinit->n_local) return -4;
+  if (!c.A && (int)c.h_x.size() != pevsl->n) {
+    c.h_x.assign((size_t)pevsl->n, 0.0);
+    c.h_y.assign((size_t)pevsl->n, 0.0);
+  }
+  BatchSolver bs(c, pevsl, nslices);
+  const int rc = bs.run(intvs, maxits, tol, vinit, pols, fstats);
+  if (rc == -1) {
+    for (int s = 0; s < nslices; s++) {
+      nevOut[s] = 0;
+      lamo[s] = reso[s] = NULL;
+      if (Wo) Wo[s] = NULL;
+    }
+    return rc;
+  }
+  bs.outputs(nevOut, lamo, Wo, reso, vinit);
+  return rc;
+}
+
+int pEVSL_ChebLanNr(pevsl_Data *pevsl, double *intv, int maxit, double tol,
+                    pevsl_Parvec *vinit, pevsl_Polparams *pol, int *nevOut,
+                    double **lamo, pevsl_Parvecs **Wo, double **reso, FILE *fstats) {
+  // invalid interval: nothing computed, return 0 (SRC/cheblanNr.c:96-100)
+  if (intv[0] >= intv[1] || intv[0] >= intv[3] || intv[1] <= intv[2]) {
+    *nevOut = 0;
+    *lamo = NULL;
+    *Wo = NULL;
+    *reso = NULL;
+    return 0;
+  }
+  if (fstats) {
+    fprintf(fstats, " intv:[%e, %e, %e, %e]\n", intv[0], intv[1], intv[2], intv[3]);
+    fprintf(fstats, " ** Cheb Poly of deg = %d, gam = %.15e, bar: %.15e\n", pol->deg,
+            pol->gam, pol->bar);
+    fprintf(fstats, " ** Cheb-LanNr \n");
+  }
+  pevsl_Parvecs *W = NULL;
+  const int rc = pEVSL_ChebLanNrBatch(pevsl, 1, intv, &maxit, tol, vinit, pol, nevOut, lamo,
+                                      &W, reso, fstats);
+  *Wo = W;
+  return rc;
+}
+
+}  // extern "C"

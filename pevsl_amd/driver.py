@@ -1,0 +1,338 @@
+"""Host driver of the spectrum-slicing run through the C API.
+
+`lap_slicing_run` follows TESTS/Lap/LapPLanN.c:15-323 for one process and one
+group: Laplacian matrix, pEVSL_Start (srand(0)), KPM DOS (Mdeg 300, Jackson,
+nvec 60) or Lanczos DOS, equal-count slicing, one random vinit shared by all
+slices, then per slice FindPol (damping 2, thresholds 0.8/0.2, max_deg 3000)
+and ChebLanNr with mlan = min(max(5 nev, 300), n), tol 1e-8.  Slices can also
+be solved together with pEVSL_ChebLanNrBatch.
+
+Every product, filter and Lanczos step runs in libpevsl_amd.so on the GPU;
+this module only prepares host arrays, as the reference's C driver does.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import capi
+
+
+def laplacian3d(nx, ny, nz):
+    """7-point (5-point for nz == 1) Dirichlet Laplacian, natural order
+    idx = k*nx*ny + i*nx + j with j fastest, entries of a row ordered
+    -z, -y, -x, diag, +x, +y, +z (TESTS/COMMON/lapl.c:4-70).
+    Returns (ia, ja, a) as int32/int32/float64 arrays."""
+    n = nx * ny * nz
+    r = np.arange(n, dtype=np.int64)
+    k = r // (nx * ny)
+    i = (r - k * nx * ny) // nx
+    j = r - k * nx * ny - i * nx
+    dg = 6.0 if nz > 1 else 4.0
+    cols, vals, ok = [], [], []
+    for cond, off, v in ((k > 0, -nx * ny, -1.0), (i > 0, -nx, -1.0),
+                         (j > 0, -1, -1.0), (np.ones(n, bool), 0, dg),
+                         (j < nx - 1, 1, -1.0), (i < ny - 1, nx, -1.0),
+                         (k < nz - 1, nx * ny, -1.0)):
+        cols.append(r + off)
+        vals.append(np.full(n, v))
+        ok.append(cond)
+    cols = np.stack(cols, 1)
+    vals = np.stack(vals, 1)
+    ok = np.stack(ok, 1)
+    counts = ok.sum(1)
+    ia = np.zeros(n + 1, dtype=np.int32)
+    np.cumsum(counts, out=ia[1:])
+    ja = cols[ok].astype(np.int32)
+    a = vals[ok].astype(np.float64)
+    return ia, ja, a
+
+
+def exact_eig_lap3(nx, ny, nz, a, b):
+    """Closed-form eigenvalues in [a - eps, b + eps], sorted
+    (TESTS/COMMON/lapl.c:139-169)."""
+    th = [0.5 * np.pi / (m + 1.0) for m in (nx, ny, nz)]
+    tx = np.sin(np.arange(1, nx + 1) * th[0])
+    ty = np.sin(np.arange(1, ny + 1) * th[1])
+    tz = np.zeros(1) if nz == 1 else np.sin(np.arange(1, nz + 1) * th[2])
+    ev = 4 * (tx[:, None, None] ** 2 + ty[None, :, None] ** 2 + tz[None, None, :] ** 2)
+    ev = ev.ravel()
+    eps = np.finfo(float).eps
+    return np.sort(ev[(ev >= a - eps) & (ev <= b + eps)])
+
+
+class Matrix:
+    """A pevsl_Parcsr built through pEVSL_ParcsrCreate/Setup (device copy)."""
+
+    def __init__(self, lib, ia, ja, a):
+        self.lib = lib
+        n = len(ia) - 1
+        self._ia = np.ascontiguousarray(ia, dtype=np.int32)
+        self._ja = np.ascontiguousarray(ja, dtype=np.int32)
+        self._a = np.ascontiguousarray(a, dtype=np.float64)
+        csr = capi.Csr(n, n, self._ia.ctypes.data_as(capi.c_int_p),
+                       self._ja.ctypes.data_as(capi.c_int_p),
+                       self._a.ctypes.data_as(capi.c_double_p))
+        self.A = capi.Parcsr()
+        rc = lib.pEVSL_ParcsrCreate(n, n, None, None, C.byref(self.A),
+                                    capi.MPI_COMM_WORLD)
+        if rc:
+            raise RuntimeError("ParcsrCreate failed: %d" % rc)
+        rc = lib.pEVSL_ParcsrSetup(C.byref(csr), C.byref(self.A))
+        if rc:
+            raise RuntimeError("ParcsrSetup failed: %d" % rc)
+        self.n = n
+        self.nnz = int(self._ia[-1])
+
+    def matvec(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        X, Y = capi.Parvec(), capi.Parvec()
+        self.lib.pEVSL_ParvecCreateShell(self.n, self.n, 0, 0, C.byref(X),
+                                         x.ctypes.data_as(capi.c_double_p))
+        self.lib.pEVSL_ParvecCreateShell(self.n, self.n, 0, 0, C.byref(Y),
+                                         y.ctypes.data_as(capi.c_double_p))
+        self.lib.pEVSL_ParcsrMatvec(C.byref(self.A), C.byref(X), C.byref(Y))
+        return y
+
+    def free(self):
+        if self.A.dev:
+            self.lib.pEVSL_ParcsrFree(C.byref(self.A))
+
+
+class Instance:
+    """pEVSL_Start / pEVSL_Finish around one matrix."""
+
+    def __init__(self, lib=None, matrix=None, prec=capi.PEVSL_PREC_FP64,
+                 rand_mode=capi.PEVSL_RAND_LIBC, seed=0, keep_vectors=False,
+                 lanes=0, block_width=0):
+        self.lib = lib or capi.load()
+        self.p = capi.DataP()
+        rc = self.lib.pEVSL_Start(capi.MPI_COMM_WORLD, C.byref(self.p))
+        if rc:
+            raise RuntimeError("pEVSL_Start failed: %d" % rc)
+        if matrix is not None:
+            self.lib.pEVSL_SetAParcsr(self.p, C.byref(matrix.A))
+        self.lib.pEVSL_SetFilterPrecision(self.p, prec)
+        self.lib.pEVSL_SetRandMode(self.p, rand_mode, seed)
+        self.lib.pEVSL_SetKeepVectors(self.p, 1 if keep_vectors else 0)
+        self.lib.pEVSL_SetSpmvTuning(self.p, lanes, block_width)
+
+    def close(self):
+        if self.p:
+            self.lib.pEVSL_Finish(self.p)
+            self.p = capi.DataP()
+
+    def kpmdos(self, Mdeg, damping, nvec, intv):
+        mu = np.zeros(Mdeg + 1)
+        ecnt = C.c_double()
+        iv = np.ascontiguousarray(intv, dtype=np.float64)
+        rc = self.lib.pEVSL_Kpmdos(self.p, Mdeg, damping, nvec,
+                                   iv.ctypes.data_as(capi.c_double_p), 1, 0, -1,
+                                   mu.ctypes.data_as(capi.c_double_p), C.byref(ecnt))
+        return rc, mu, ecnt.value
+
+    def landos(self, nvec, msteps, npts, intv):
+        xd = np.zeros(npts)
+        yd = np.zeros(npts)
+        neig = C.c_double()
+        iv = np.ascontiguousarray(intv, dtype=np.float64)
+        rc = self.lib.pEVSL_LanDosG(self.p, nvec, msteps, npts,
+                                    xd.ctypes.data_as(capi.c_double_p),
+                                    yd.ctypes.data_as(capi.c_double_p),
+                                    C.byref(neig), iv.ctypes.data_as(capi.c_double_p),
+                                    1, 0, -1)
+        return rc, xd, yd, neig.value
+
+    def kernel_stats(self):
+        ks = capi.KernelStats()
+        self.lib.pEVSL_GetKernelStats(self.p, C.byref(ks))
+        return {f: getattr(ks, f) for f, _ in capi.KernelStats._fields_}
+
+    def stats_reset(self):
+        self.lib.pEVSL_StatsReset(self.p)
+
+    def lan_info(self, s=0):
+        li = capi.LanInfo()
+        if self.lib.pEVSL_GetLanInfo(self.p, s, C.byref(li)):
+            return None
+        return {f: getattr(li, f) for f, _ in capi.LanInfo._fields_}
+
+
+def spslicer_kpm(lib, mu, intv, nslices, npts):
+    sli = np.zeros(nslices + 1)
+    iv = np.ascontiguousarray(intv, dtype=np.float64)
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    rc = lib.pEVSL_SpslicerKpm(sli.ctypes.data_as(capi.c_double_p),
+                               mu.ctypes.data_as(capi.c_double_p), len(mu) - 1,
+                               iv.ctypes.data_as(capi.c_double_p), nslices, npts)
+    return rc, sli
+
+
+def spslicer_lan(lib, xd, yd, nslices):
+    sli = np.zeros(nslices + 1)
+    xd = np.ascontiguousarray(xd, dtype=np.float64)
+    yd = np.array(yd, dtype=np.float64)
+    lib.pEVSL_SpslicerLan(xd.ctypes.data_as(capi.c_double_p),
+                          yd.ctypes.data_as(capi.c_double_p), nslices, len(xd),
+                          sli.ctypes.data_as(capi.c_double_p))
+    return sli
+
+
+def find_pol(lib, intv, damping=2, thresh_int=0.8, thresh_ext=0.2, max_deg=3000,
+             deg=0):
+    """pEVSL_SetPolDef + the driver's overrides + pEVSL_FindPol.  Returns the
+    Polparams (caller frees with lib.pEVSL_FreePol) and the return code."""
+    pol = capi.Polparams()
+    lib.pEVSL_SetPolDef(C.byref(pol))
+    pol.damping = damping
+    pol.thresh_int = thresh_int
+    pol.thresh_ext = thresh_ext
+    pol.max_deg = max_deg
+    pol.deg = deg
+    iv = np.ascontiguousarray(intv, dtype=np.float64)
+    rc = lib.pEVSL_FindPol(iv.ctypes.data_as(capi.c_double_p), C.byref(pol))
+    return pol, rc
+
+
+def pol_mu(pol):
+    return np.ctypeslib.as_array(pol.mu, shape=(pol.deg + 1,)).copy()
+
+
+def rand_vinit(lib, n):
+    """vinit by pEVSL_ParvecRand (libc rand(), SRC/parvec.c:65-72)."""
+    v = np.empty(n)
+    pv = capi.Parvec()
+    lib.pEVSL_ParvecCreateShell(n, n, 0, 0, C.byref(pv), v.ctypes.data_as(capi.c_double_p))
+    lib.pEVSL_ParvecRand(C.byref(pv))
+    return v
+
+
+def cheblan_batch(inst, intvs, maxits, tol, vinit, pols, keep_vectors=False):
+    """pEVSL_ChebLanNrBatch over len(pols) slices.  Returns a list of dicts
+    with lam (unsorted, reference order), res, vecs (or None) and info."""
+    lib = inst.lib
+    ns = len(pols)
+    iv = np.ascontiguousarray(np.asarray(intvs, dtype=np.float64).reshape(ns, 4))
+    mi = np.ascontiguousarray(maxits, dtype=np.int32)
+    n = len(vinit)
+    vin = np.ascontiguousarray(vinit, dtype=np.float64)
+    pv = capi.Parvec()
+    lib.pEVSL_ParvecCreateShell(n, n, 0, 0, C.byref(pv), vin.ctypes.data_as(capi.c_double_p))
+    parr = (capi.Polparams * ns)(*pols)
+    nev = np.zeros(ns, dtype=np.int32)
+    lamo = (capi.c_double_p * ns)()
+    reso = (capi.c_double_p * ns)()
+    Wo = (C.POINTER(capi.Parvecs) * ns)()
+    lib.pEVSL_SetKeepVectors(inst.p, 1 if keep_vectors else 0)
+    rc = lib.pEVSL_ChebLanNrBatch(inst.p, ns, iv.ctypes.data_as(capi.c_double_p),
+                                  mi.ctypes.data_as(capi.c_int_p), tol, C.byref(pv),
+                                  parr, nev.ctypes.data_as(capi.c_int_p), lamo, Wo,
+                                  reso, None)
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    out = []
+    for s in range(ns):
+        k = int(nev[s])
+        lam = np.ctypeslib.as_array(lamo[s], shape=(max(k, 1),))[:k].copy() if lamo[s] else np.zeros(0)
+        res = np.ctypeslib.as_array(reso[s], shape=(max(k, 1),))[:k].copy() if reso[s] else np.zeros(0)
+        vecs = None
+        if Wo[s]:
+            W = Wo[s].contents
+            if k:
+                vecs = np.ctypeslib.as_array(W.data, shape=(k, W.ld))[:, :n].copy()
+            lib.pEVSL_ParvecsFree(Wo[s])
+            libc.free(C.cast(Wo[s], C.c_void_p))
+        if lamo[s]:
+            libc.free(C.cast(lamo[s], C.c_void_p))
+        if reso[s]:
+            libc.free(C.cast(reso[s], C.c_void_p))
+        out.append({"lam": lam, "res": res, "vecs": vecs, "info": inst.lan_info(s)})
+    return rc, out
+
+
+def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
+                    prec=capi.PEVSL_PREC_FP64, batch=False, slices_only=None,
+                    keep_vectors=False, rand_mode=capi.PEVSL_RAND_LIBC, seed=0,
+                    sli_override=None, block_width=0, log=None):
+    """End-to-end run of the TESTS/Lap/LapPLanN.c flow on the GPU.  Returns a
+    dict with the DOS, slices, per-slice filters, eigenvalues and timings."""
+    lib = capi.load()
+    t0 = time.time()
+    ia, ja, av = laplacian3d(nx, ny, nz)
+    M = Matrix(lib, ia, ja, av)
+    n = M.n
+    lmin, lmax = 0.0, (8.0 if nz == 1 else 12.0)
+    tol = 1e-8
+    inst = Instance(lib, M, prec=prec, rand_mode=rand_mode, seed=seed,
+                    block_width=block_width)
+    out = {"n": n, "nnz": M.nnz, "t_setup": time.time() - t0}
+    try:
+        xintv = np.array([a, b, lmin, lmax])
+        t1 = time.time()
+        if dostype == 0:
+            rc, mu, ecount = inst.kpmdos(Mdeg, 1, nvec, xintv)
+            if rc:
+                raise RuntimeError("kpmdos error %d" % rc)
+            npts = int(10 * ecount)
+            rc, sli = spslicer_kpm(lib, mu, xintv, nslices, npts)
+            if rc:
+                raise RuntimeError("spslicer error %d" % rc)
+            out["mu"] = mu
+        else:
+            npts = 200
+            rc, xd, yd, ecount = inst.landos(nvec, 40, npts, xintv)
+            sli = spslicer_lan(lib, xd, yd, nslices)
+        out["t_dos"] = time.time() - t1
+        if sli_override is not None:
+            sli = np.asarray(sli_override, dtype=np.float64)
+        out["ecount"] = ecount
+        out["sli"] = sli
+        ev_int = int(1 + ecount / float(nslices))
+        vinit = rand_vinit(lib, n)
+        todo = list(range(nslices)) if slices_only is None else list(slices_only)
+        nev = ev_int + 2
+        mlan = min(max(5 * nev, 300), n)
+        pols, intvs = [], []
+        t2 = time.time()
+        for sl in todo:
+            iv = [sli[sl], sli[sl + 1], lmin, lmax]
+            pol, rc = find_pol(lib, iv)
+            pols.append(pol)
+            intvs.append(iv)
+        out["t_findpol"] = time.time() - t2
+        res_slices = []
+        t3 = time.time()
+        groups = [list(range(len(todo)))] if batch else [[i] for i in range(len(todo))]
+        for g in groups:
+            rc, outs = cheblan_batch(inst, [intvs[i] for i in g], [mlan] * len(g), tol,
+                                     vinit, [pols[i] for i in g], keep_vectors)
+            if rc:
+                raise RuntimeError("ChebLanNr error %d" % rc)
+            for i, o in zip(g, outs):
+                sl = todo[i]
+                lam_sorted = np.sort(o["lam"])
+                ex = exact_eig_lap3(nx, ny, nz, sli[sl], sli[sl + 1])
+                d = {"slice": sl, "intv": [float(sli[sl]), float(sli[sl + 1])],
+                     "deg": pols[i].deg, "type": pols[i].type, "bar": pols[i].bar,
+                     "gam": pols[i].gam, "mlan": mlan, "nev": len(o["lam"]),
+                     "nev_exact": len(ex), "lam": lam_sorted, "lam_raw": o["lam"],
+                     "res": o["res"], "exact": ex, "info": o["info"],
+                     "vecs": o["vecs"]}
+                if len(ex) == len(lam_sorted) and len(ex):
+                    d["max_rel_err"] = float(np.max(np.abs(lam_sorted - ex) / np.abs(ex)))
+                res_slices.append(d)
+                if log:
+                    log("slice %d: deg %d its %s nev %d exact %d" % (
+                        sl, pols[i].deg, o["info"]["its"] if o["info"] else "?",
+                        len(o["lam"]), len(ex)))
+        out["t_lanczos"] = time.time() - t3
+        out["slices"] = res_slices
+        out["kstats"] = inst.kernel_stats()
+        for p_ in pols:
+            lib.pEVSL_FreePol(C.byref(p_))
+    finally:
+        inst.close()
+        M.free()
+    out["t_total"] = time.time() - t0
+    return out
