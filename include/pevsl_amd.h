@@ -42,6 +42,9 @@ typedef struct {
   double reorth_bytes;    /* algorithmic bytes of the reorthogonalization */
   long long filter_steps; /* column-degree steps                        */
   long long filter_launches;
+  double t_cgs;           /* Gram-Schmidt passes only (GEMV-T + GEMV-N)  */
+  double cgs_bytes;       /* algorithmic bytes of those passes           */
+  long long cgs_passes;   /* passes run (first + DGKS second passes)     */
 } pevsl_KernelStats;
 
 /* Filter precision: FP64 (reference) or FP32 vectors and matrix values with
@@ -50,6 +53,13 @@ typedef struct {
 #define PEVSL_PREC_FP64 0
 #define PEVSL_PREC_FP32 1
 int pEVSL_SetFilterPrecision(pevsl_Data *pevsl, int prec);
+
+/* Residual threshold used instead of ChebLanNr's tol when the filter runs in
+ * FP32 (default 1e-5): Ritz vectors built from an FP32-filtered Krylov basis
+ * carry FP32-level errors (~1e-6 relative), while their FP64 Rayleigh
+ * quotients stay accurate to ~1e-12.  The accepted set and the reported
+ * residuals use this threshold; eigenvalue errors are reported separately. */
+int pEVSL_SetMixedTol(pevsl_Data *pevsl, double tol);
 
 /* Random vectors: PEVSL_RAND_LIBC draws (rand()-RAND_MAX/2)/(RAND_MAX/2) from
  * libc on the host in the reference order (SRC/parvec.c:65-72);
@@ -79,6 +89,13 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs,
                          pevsl_Polparams *pols, int *nevOut, double **lamo,
                          pevsl_Parvecs **Wo, double **reso, FILE *fstats);
 
+/* y = p((A - cc)/dd) v for host vectors v, y (w unused: the work vectors
+ * live on the device).  Same arguments as the reference's internal
+ * pEVSL_ChebAv (INC/pevsl_int.h:25, SRC/chebpol.c:509-574); exported for
+ * callers that apply a designed filter outside the Lanczos solve. */
+int pEVSL_ChebAv(pevsl_Data *pevsl, pevsl_Polparams *pol, pevsl_Parvec *v,
+                 pevsl_Parvec *y, pevsl_Parvec *w);
+
 /* Diagnostics of slice s of the last ChebLanNr / ChebLanNrBatch call. */
 int pEVSL_GetLanInfo(pevsl_Data *pevsl, int s, pevsl_LanInfo *info);
 int pEVSL_GetKernelStats(pevsl_Data *pevsl, pevsl_KernelStats *st);
@@ -89,6 +106,31 @@ int pEVSL_DeviceMatvec(pevsl_Data *pevsl, const double *d_x, double *d_y);
 
 /* Device name and HBM size of the current device; 0 on success. */
 int pEVSL_DeviceInfo(char *name, int len, size_t *hbm_bytes);
+
+/* Autotuner of the fused filter step on the instance's matrix.  Times
+ * `reps` back-to-back Chebyshev steps (HIP events) for every candidate and
+ * records the fastest:
+ *   one column: lanes per row in {1,2,4,8,16,32}            -> lanes
+ *   block SpMM: widths {2,4,8,16,32,64}, per-column cost     -> block_width
+ * for the instance's filter precision, and installs the choice (as
+ * pEVSL_SetSpmvTuning).  Results: seconds per step of each candidate
+ * (t_lanes[i] for lanes 2^i, t_block[i] for width 2^(i+1)) and the
+ * algorithmic bytes of one step (bytes_lanes, bytes_block[i]). */
+typedef struct {
+  int lanes;
+  int block_width;
+  double t_lanes[6];
+  double bytes_lanes;
+  double t_block[6];
+  double bytes_block[6];
+} pevsl_TuneResult;
+int pEVSL_Autotune(pevsl_Data *pevsl, int reps, pevsl_TuneResult *res);
+
+/* Times `reps` classical Gram-Schmidt passes of one vector against a basis
+ * of k FP64 columns (the reorthogonalization kernels); returns seconds per
+ * pass and its algorithmic bytes (2 reads of the basis + vector traffic). */
+int pEVSL_BenchReorth(pevsl_Data *pevsl, int k, int reps, double *sec,
+                      double *bytes);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,10 @@
  * Options: -nx -ny -nz -a -b -nslices -dostype -nvec -Mdeg -seed
  *          -slice s (run only slice s; -1 = all)
  *          -maxsteps K (cap Lanczos steps, for bounded CPU-baseline samples)
+ *          -dostype 2: no DOS estimate; one slice [a, b] (nslices forced to
+ *                      1), nev taken as -nev (timing samples)
+ *          -reorthk K: also time three CGS-DGKS passes against K basis
+ *                      columns ("t_cgs_pass" in the output)
  */
 #include <float.h>
 #include <math.h>
@@ -55,6 +59,8 @@ int main(int argc, char **argv) {
   int seed = argi(argc, argv, "seed", 0);
   int only = argi(argc, argv, "slice", -1);
   int maxsteps = argi(argc, argv, "maxsteps", 0);
+  int reorthk = argi(argc, argv, "reorthk", 0);
+  if (dostype == 2) nslices = 1;
   double lmin = 0.0, lmax = nz == 1 ? 8.0 : 12.0, tol = 1e-8;
   int n = nx * ny * nz;
   double xintv[4] = {a, b, lmin, lmax};
@@ -73,6 +79,10 @@ int main(int argc, char **argv) {
     int npts = (int)(10 * ecount);
     ierr = orc_spslicer_kpm(sli, mu, Mdeg, xintv, nslices, npts);
     if (ierr) { fprintf(stderr, "spslicer error %d\n", ierr); return 1; }
+  } else if (dostype == 2) {
+    sli[0] = a;
+    sli[1] = b;
+    ecount = argi(argc, argv, "nev", 100);
   } else {
     int msteps = 40, npts = 200;
     double *xd = (double *)calloc(npts, sizeof(double));
@@ -148,7 +158,24 @@ int main(int argc, char **argv) {
     free(ind);
     orc_free_pol(&pol);
   }
-  printf("], \"t_total\": %.6f}\n", wtime() - t0);
+  printf("]");
+  if (reorthk > 0) {
+    double *V = (double *)malloc(sizeof(double) * (size_t)n * reorthk);
+    double *v = (double *)malloc(sizeof(double) * (size_t)n);
+    double *w = (double *)malloc(sizeof(double) * (size_t)reorthk);
+    orc_rand_vec(n, v);
+    for (size_t i = 0; i < (size_t)n * reorthk; i++) V[i] = 1e-3 * v[i % n] + 1e-6;
+    double nrm, tc = wtime();
+    for (int r = 0; r < 3; r++) {
+      orc_rand_vec(n, v);
+      orc_cgs_dgks(n, reorthk, 1, V, n, v, &nrm, w);
+    }
+    printf(", \"reorthk\": %d, \"t_cgs_pass\": %.6f", reorthk, (wtime() - tc) / 3);
+    free(V);
+    free(v);
+    free(w);
+  }
+  printf(", \"t_total\": %.6f}\n", wtime() - t0);
   free(mu);
   free(sli);
   free(vinit);

@@ -363,6 +363,7 @@ int orc_kpmdos(const orc_csr *A, int Mdeg, int damping, int nvec,
  * in closed form: SRC/spslicer.c:12-38. */
 void orc_intchx(int Mdeg, const double *mu, int npts, const double *xi,
                 double *yi) {
+  if (npts <= 0) return;
   double *th = (double *)xmalloc(sizeof(double) * (size_t)npts);
   for (int j = 0; j < npts; j++) th[j] = acos(xi[j]);
   double th0 = th[0];

@@ -81,12 +81,19 @@ class LanInfo(C.Structure):
                 ("t_total", C.c_double)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("lanes", C.c_int), ("block_width", C.c_int),
+                ("t_lanes", C.c_double * 6), ("bytes_lanes", C.c_double),
+                ("t_block", C.c_double * 6), ("bytes_block", C.c_double * 6)]
+
+
 class KernelStats(C.Structure):
     _fields_ = [("t_filter", C.c_double), ("t_reorth", C.c_double),
                 ("t_eig", C.c_double), ("t_ritz", C.c_double),
                 ("t_kpm", C.c_double), ("filter_bytes", C.c_double),
                 ("reorth_bytes", C.c_double), ("filter_steps", C.c_longlong),
-                ("filter_launches", C.c_longlong)]
+                ("filter_launches", C.c_longlong), ("t_cgs", C.c_double),
+                ("cgs_bytes", C.c_double), ("cgs_passes", C.c_longlong)]
 
 
 DataP = C.POINTER(Data)
@@ -201,6 +208,7 @@ SIGNATURES = {
                                   C.POINTER(c_double_p), C.c_void_p]),
     # pevsl_amd.h extensions
     "pEVSL_SetFilterPrecision": (C.c_int, [DataP, C.c_int]),
+    "pEVSL_SetMixedTol": (C.c_int, [DataP, C.c_double]),
     "pEVSL_SetRandMode": (C.c_int, [DataP, C.c_int, C.c_ulonglong]),
     "pEVSL_SetKeepVectors": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetSpmvTuning": (C.c_int, [DataP, C.c_int, C.c_int]),
@@ -210,10 +218,14 @@ SIGNATURES = {
                                        C.POINTER(c_double_p),
                                        C.POINTER(C.POINTER(Parvecs)),
                                        C.POINTER(c_double_p), C.c_void_p]),
+    "pEVSL_ChebAv": (C.c_int, [DataP, C.POINTER(Polparams), C.POINTER(Parvec),
+                               C.POINTER(Parvec), C.POINTER(Parvec)]),
     "pEVSL_GetLanInfo": (C.c_int, [DataP, C.c_int, C.POINTER(LanInfo)]),
     "pEVSL_GetKernelStats": (C.c_int, [DataP, C.POINTER(KernelStats)]),
     "pEVSL_DeviceMatvec": (C.c_int, [DataP, C.c_void_p, C.c_void_p]),
     "pEVSL_DeviceInfo": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_size_t)]),
+    "pEVSL_Autotune": (C.c_int, [DataP, C.c_int, C.POINTER(TuneResult)]),
+    "pEVSL_BenchReorth": (C.c_int, [DataP, C.c_int, C.c_int, c_double_p, c_double_p]),
 }
 
 _lib = None

@@ -45,9 +45,10 @@ const DevCsr *dev_matrix_of(pevsl_Parcsr *A) {
   return &static_cast<DevMatrix *>(A->dev)->csr;
 }
 
+// about three nonzeros per lane (measured best for 7-point stencils: 2)
 int auto_lanes(double mean_row) {
   int L = 1;
-  while (L < 64 && L < mean_row * 0.75) L *= 2;
+  while (L < 64 && 2 * L <= mean_row / 3.0) L *= 2;
   return L;
 }
 
@@ -218,6 +219,12 @@ int pEVSL_LanTrbounds(pevsl_Data *, int, int, double, pevsl_Parvec *, int,
 int pEVSL_SetFilterPrecision(pevsl_Data *p, int prec) {
   if (prec != PEVSL_PREC_FP64 && prec != PEVSL_PREC_FP32) return -1;
   ctx_of(p).filter_prec = prec;
+  return 0;
+}
+
+int pEVSL_SetMixedTol(pevsl_Data *p, double tol) {
+  if (!(tol > 0)) return -1;
+  ctx_of(p).mixed_tol = tol;
   return 0;
 }
 

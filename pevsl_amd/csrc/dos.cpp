@@ -296,13 +296,14 @@ int pEVSL_LanDosG(pevsl_Data *pevsl, const int nvec, int msteps, int npts, doubl
   L.ld = ((size_t)n + 63) / 64 * 64;
   L.nslice = nvec;
   int shift = 0;
-  while ((1 << shift) < maxit + 1) shift++;
+  while ((1 << shift) < maxit + 1 || shift < 5) shift++;
   L.chunk_shift = shift;
   L.maxchunks = 1;
   DBuf<double> V, sc, h, part, raw;
   DBuf<double *> vtab;
   DBuf<int> fl;
   V.ensure(L.ld * ((size_t)1 << shift) * nvec);
+  PV_HIP(hipMemsetAsync(V.p, 0, sizeof(double) * L.ld * ((size_t)1 << shift) * nvec, s));
   std::vector<double *> vt((size_t)nvec);
   for (int v = 0; v < nvec; v++) vt[(size_t)v] = V.p + L.ld * ((size_t)1 << shift) * v;
   L.vtab = vtab.ensure((size_t)nvec);

@@ -75,6 +75,7 @@ struct Ctx {
   MVFunc host_mv = nullptr;        // set by SetAMatvec (host callback)
   void *host_mv_data = nullptr;
   int filter_prec = PEVSL_PREC_FP64;
+  double mixed_tol = 1e-5;         // residual threshold of the FP32-filter path
   int rand_mode = PEVSL_RAND_LIBC;
   unsigned long long rand_seed = 0;
   unsigned long long rand_offset = 0;

@@ -96,23 +96,28 @@ k_cheb_vec(int n, const int *__restrict__ ia, const int *__restrict__ ja,
   const size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x;
   const int row = (int)(g / L);
   const int lane = (int)(g % L);
+  // all lanes of a row group exit together, so the group shuffles below
+  // only involve active lanes
+  if (row >= n) return;
+  const int b = ia[row], e = ia[row + 1];
+  // operands of the update are loaded before the row sum so that their
+  // latency overlaps the gather
+  const T v = vk[row];
+  const T vm = first ? (T)0 : vkm1[row];
+  const T yv = first ? (T)0 : y[row];
   T acc = 0;
-  if (row < n) {
-    const int e = ia[row + 1];
-    for (int j = ia[row] + lane; j < e; j += L) acc += a[j] * vk[ja[j]];
-  }
+  for (int j = b + lane; j < e; j += L) acc += a[j] * vk[ja[j]];
   if (L > 1) acc = group_sum<L>(acc);
-  if (row < n && lane == 0) {
-    const T v = vk[row];
+  if (lane == 0) {
     T w = acc + (-cc) * v;  // A vk - cc vk
     w = w * t;
     if (first) {
       vkp1[row] = w;
       y[row] = v * mu0 + muk * w;
     } else {
-      w = w + (T)(-1) * vkm1[row];
+      w = w + (T)(-1) * vm;
       vkp1[row] = w;
-      y[row] = y[row] + muk * w;
+      y[row] = yv + muk * w;
     }
   }
 }
@@ -196,14 +201,22 @@ k_kpm_blk(int n, const int *__restrict__ ia, const int *__restrict__ ja,
 
 constexpr int kDotBlocksMax = 1024;
 
+// out[c] = sum_b part[b*ncol + c]; one block per column, fixed order
 __global__ void __launch_bounds__(kThreads)
 k_reduce_partials(const double *__restrict__ part, int nblk, int ncol,
                   double *__restrict__ out) {
-  const int c = blockIdx.x * kThreads + threadIdx.x;
-  if (c >= ncol) return;
+  const int c = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nblk; b++) s += part[(size_t)b * ncol + c];
-  out[c] = s;
+  for (int b = threadIdx.x; b < nblk; b += kThreads) s += part[(size_t)b * ncol + c];
+  __shared__ double red[kThreads / 64];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kThreads / 64; w++) t += red[w];
+    out[c] = t;
+  }
 }
 
 // Block-level sum of v into part[blockIdx.x] (all threads participate).
@@ -390,11 +403,15 @@ __global__ void k_lan_reduce(LanArgs L, int nblk, int which) {
   }
 }
 
-// GEMV-T partials: part[(s*hcap + j)*nrb + rb] = sum over the row block of
-// V_s(:,j) * znew_s, for j in this block's column chunk.
-constexpr int kGtRpt = 8;            // rows per thread
-constexpr int kGtRows = kThreads * kGtRpt;
-constexpr int kGtCols = 64;          // columns per block
+// GEMV-T partials: part[(s*hcap + j)*nrb + rb] = sum over row block rb of
+// V_s(:,j) * znew_s, for the kTcols columns of this block.  Each lane keeps
+// one accumulator per column and streams row pairs (16-byte loads, one
+// independent load per column per iteration); the cross-lane reduction runs
+// once per column at the end.  Rows in [n, ld) are zero in every basis
+// column, so whole row pairs up to ld are read without masks.
+constexpr int kTcols = 32;               // columns per block
+constexpr int kTiters = 16;              // row pairs per thread
+constexpr int kGtRows = kThreads * 2 * kTiters;  // rows per block (8192)
 
 __global__ void __launch_bounds__(kThreads)
 k_lan_gemvt(LanArgs L, int k, int pass, int nrb) {
@@ -402,58 +419,52 @@ k_lan_gemvt(LanArgs L, int k, int pass, int nrb) {
   if (!L.active[s]) return;
   if (pass && !L.pass2[s]) return;
   const int ncols = k + 1;
-  const int j0 = blockIdx.y * kGtCols;
+  const int j0 = blockIdx.y * kTcols;
   if (j0 >= ncols) return;
-  const int jn = min(kGtCols, ncols - j0);
+  const int jn = min(kTcols, ncols - j0);
   const int rb = blockIdx.x;
-  const double *z = lan_col(L, s, k + 1);
-  const int row0 = rb * kGtRows + threadIdx.x;
-  double zr[kGtRpt];
+  const double2 *z = reinterpret_cast<const double2 *>(lan_col(L, s, k + 1));
+  // the kTcols columns of a block lie in one basis chunk (chunks hold a
+  // power of two >= kTcols columns), kTcols-aligned: column c is at
+  // base + c*ld
+  const double2 *base = reinterpret_cast<const double2 *>(lan_col(L, s, j0));
+  const size_t cstride = L.ld / 2;
+  double acc[kTcols];
 #pragma unroll
-  for (int i = 0; i < kGtRpt; i++) {
-    const int r = row0 + i * kThreads;
-    zr[i] = r < L.n ? z[r] : 0.0;
-  }
-  __shared__ double red[kThreads / 64][kGtCols];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int jj = 0;
-  for (; jj + 4 <= jn; jj += 4) {
-    const double *v0 = lan_col(L, s, j0 + jj);
-    const double *v1 = lan_col(L, s, j0 + jj + 1);
-    const double *v2 = lan_col(L, s, j0 + jj + 2);
-    const double *v3 = lan_col(L, s, j0 + jj + 3);
-    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  for (int c = 0; c < kTcols; c++) acc[c] = 0.0;
+  const size_t npair = L.ld / 2;
+  const size_t p0 = (size_t)rb * (kGtRows / 2) + threadIdx.x;
+  if (jn == kTcols) {
+    for (int i = 0; i < kTiters; i++) {
+      const size_t p = p0 + (size_t)i * kThreads;
+      if (p >= npair) break;
+      const double2 zz = z[p];
 #pragma unroll
-    for (int i = 0; i < kGtRpt; i++) {
-      const int r = row0 + i * kThreads;
-      if (r < L.n) {
-        a0 += v0[r] * zr[i];
-        a1 += v1[r] * zr[i];
-        a2 += v2[r] * zr[i];
-        a3 += v3[r] * zr[i];
+      for (int c = 0; c < kTcols; c++) {
+        const double2 v = base[c * cstride + p];
+        acc[c] += v.x * zz.x + v.y * zz.y;
       }
     }
-    a0 = wave_sum(a0);
-    a1 = wave_sum(a1);
-    a2 = wave_sum(a2);
-    a3 = wave_sum(a3);
-    if (lane == 0) {
-      red[wave][jj] = a0;
-      red[wave][jj + 1] = a1;
-      red[wave][jj + 2] = a2;
-      red[wave][jj + 3] = a3;
+  } else {
+    for (int i = 0; i < kTiters; i++) {
+      const size_t p = p0 + (size_t)i * kThreads;
+      if (p >= npair) break;
+      const double2 zz = z[p];
+#pragma unroll
+      for (int c = 0; c < kTcols; c++) {
+        if (c < jn) {
+          const double2 v = base[c * cstride + p];
+          acc[c] += v.x * zz.x + v.y * zz.y;
+        }
+      }
     }
   }
-  for (; jj < jn; jj++) {
-    const double *v0 = lan_col(L, s, j0 + jj);
-    double a0 = 0;
+  __shared__ double red[kThreads / 64][kTcols];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < kGtRpt; i++) {
-      const int r = row0 + i * kThreads;
-      if (r < L.n) a0 += v0[r] * zr[i];
-    }
-    a0 = wave_sum(a0);
-    if (lane == 0) red[wave][jj] = a0;
+  for (int c = 0; c < kTcols; c++) {
+    const double t = wave_sum(acc[c]);
+    if (lane == 0) red[wave][c] = t;
   }
   __syncthreads();
   if (threadIdx.x < jn) {
@@ -478,37 +489,73 @@ k_lan_reduce_h(LanArgs L, int k, int pass, int nrb) {
   L.h[(size_t)s * L.hcap + j] = v;
 }
 
-// znew -= V(:,0:k) h, column by column (the order of DGEMV 'N'), with the
-// block partial of |znew|^2.
-constexpr int kGnRpt = 4;
+// znew -= V(:,0:k) h, column by column in the order of DGEMV 'N'
+// (y_i += (-h_j) V_ij for j = 0..k), with the block partial of |znew|^2.
+// Each thread owns two row pairs (16-byte loads) and walks the columns
+// eight at a time so that 16 loads are in flight per thread.
+constexpr int kGnPairs = 2;                          // row pairs per thread
+constexpr int kGnRows = kThreads * 2 * kGnPairs;     // rows per block (1024)
+constexpr int kGnUnroll = 8;
+
 __global__ void __launch_bounds__(kThreads)
 k_lan_gemvn(LanArgs L, int k, int pass, int nblk) {
 #pragma clang fp contract(off)
   const int s = blockIdx.y;
   if (!L.active[s]) return;
   if (pass && !L.pass2[s]) return;
-  double *z = lan_col(L, s, k + 1);
+  double2 *z = reinterpret_cast<double2 *>(lan_col(L, s, k + 1));
   const double *h = L.h + (size_t)s * L.hcap;
-  double acc[kGnRpt];
-  int rows[kGnRpt];
+  const size_t npair = L.ld / 2;
+  size_t pr[kGnPairs];
+  double2 acc[kGnPairs];
 #pragma unroll
-  for (int i = 0; i < kGnRpt; i++) {
-    rows[i] = (blockIdx.x * kGnRpt + i) * kThreads + threadIdx.x;
-    acc[i] = rows[i] < L.n ? z[rows[i]] : 0.0;
+  for (int i = 0; i < kGnPairs; i++) {
+    pr[i] = (size_t)blockIdx.x * (kGnRows / 2) + (size_t)i * kThreads + threadIdx.x;
+    acc[i] = pr[i] < npair ? z[pr[i]] : make_double2(0.0, 0.0);
   }
-  for (int j = 0; j <= k; j++) {
-    const double *v = lan_col(L, s, j);
-    const double mh = -h[j];
+  const bool ok0 = pr[0] < npair, ok1 = pr[1] < npair;
+  int j = 0;
+  const size_t cstride = L.ld / 2;
+  for (; j + kGnUnroll <= k + 1; j += kGnUnroll) {
+    double2 v0[kGnUnroll], v1[kGnUnroll];
+    double mh[kGnUnroll];
+    // j is a multiple of kGnUnroll: the group lies in one basis chunk
+    const double2 *base = reinterpret_cast<const double2 *>(lan_col(L, s, j));
 #pragma unroll
-    for (int i = 0; i < kGnRpt; i++)
-      if (rows[i] < L.n) acc[i] = acc[i] + mh * v[rows[i]];
+    for (int u = 0; u < kGnUnroll; u++) {
+      const double2 *col = base + u * cstride;
+      mh[u] = -h[j + u];
+      v0[u] = ok0 ? col[pr[0]] : make_double2(0.0, 0.0);
+      v1[u] = ok1 ? col[pr[1]] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < kGnUnroll; u++) {
+      acc[0].x = acc[0].x + mh[u] * v0[u].x;
+      acc[0].y = acc[0].y + mh[u] * v0[u].y;
+      acc[1].x = acc[1].x + mh[u] * v1[u].x;
+      acc[1].y = acc[1].y + mh[u] * v1[u].y;
+    }
+  }
+  for (; j <= k; j++) {
+    const double2 *col = reinterpret_cast<const double2 *>(lan_col(L, s, j));
+    const double m = -h[j];
+    if (ok0) {
+      const double2 v = col[pr[0]];
+      acc[0].x = acc[0].x + m * v.x;
+      acc[0].y = acc[0].y + m * v.y;
+    }
+    if (ok1) {
+      const double2 v = col[pr[1]];
+      acc[1].x = acc[1].x + m * v.x;
+      acc[1].y = acc[1].y + m * v.y;
+    }
   }
   double sq = 0.0;
 #pragma unroll
-  for (int i = 0; i < kGnRpt; i++) {
-    if (rows[i] < L.n) {
-      z[rows[i]] = acc[i];
-      sq += acc[i] * acc[i];
+  for (int i = 0; i < kGnPairs; i++) {
+    if (pr[i] < npair) {
+      z[pr[i]] = acc[i];
+      sq += acc[i].x * acc[i].x + acc[i].y * acc[i].y;
     }
   }
   block_sum_store(sq, L.part + (size_t)s * nblk + blockIdx.x);
@@ -816,8 +863,9 @@ template int kpm_step_blk<float>(const DevCsr &, int, const float *,
 
 void reduce_partials(const double *part, int nblk, int ncol, double *out,
                      hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_partials, dim3(grid_for(ncol)), dim3(kThreads),
-                     0, s, part, nblk, ncol, out);
+  if (ncol <= 0) return;
+  hipLaunchKernelGGL(k_reduce_partials, dim3(ncol), dim3(kThreads), 0, s, part,
+                     nblk, ncol, out);
 }
 
 int dot_part_blocks(int n) { return lan_vec_blocks(n); }
@@ -880,8 +928,9 @@ template void unpack_cols<float>(int, int, int, const float *, double *const *,
                                  hipStream_t);
 
 size_t lan_part_doubles(int n, int nslice, int hcap) {
-  const size_t nrb = (size_t)(n + kGtRows - 1) / kGtRows;
-  const size_t nbn = (size_t)(n + kThreads * kGnRpt - 1) / (kThreads * kGnRpt);
+  const size_t ld = ((size_t)n + 63) / 64 * 64;
+  const size_t nrb = (ld + kGtRows - 1) / kGtRows;
+  const size_t nbn = (ld + kGnRows - 1) / kGnRows;
   const size_t per = std::max(std::max((size_t)hcap * nrb, nbn),
                               (size_t)lan_vec_blocks(n));
   return (size_t)nslice * per + 64;
@@ -905,13 +954,13 @@ void lan_subtract_alpha(const LanBatch &L, int k, hipStream_t s) {
 
 void lan_cgs_pass(const LanBatch &L, int k, int pass, hipStream_t s) {
   LanArgs a = make_args(L);
-  const int nrb = (L.n + kGtRows - 1) / kGtRows;
-  const int ncb = (k + 1 + kGtCols - 1) / kGtCols;
+  const int nrb = (int)((L.ld + kGtRows - 1) / kGtRows);
+  const int ncb = (k + 1 + kTcols - 1) / kTcols;
   hipLaunchKernelGGL(k_lan_gemvt, dim3(nrb, ncb, L.nslice), dim3(kThreads), 0,
                      s, a, k, pass, nrb);
   hipLaunchKernelGGL(k_lan_reduce_h, dim3(grid_for(k + 1), L.nslice),
                      dim3(kThreads), 0, s, a, k, pass, nrb);
-  const int nbn = (L.n + kThreads * kGnRpt - 1) / (kThreads * kGnRpt);
+  const int nbn = (int)((L.ld + kGnRows - 1) / kGnRows);
   hipLaunchKernelGGL(k_lan_gemvn, dim3(nbn, L.nslice), dim3(kThreads), 0, s, a,
                      k, pass, nbn);
   hipLaunchKernelGGL(k_lan_reduce, dim3(L.nslice), dim3(64), 0, s, a, nbn,

@@ -78,8 +78,9 @@ void unpack_cols(int n, int B, int ncol, const T *blk, double *const *cols,
                  hipStream_t s);
 
 // ---- Lanczos on a batch of slices ----------------------------------------
-// Slices are independent; slice s owns V_s (chunked columns, leading dim ld)
-// and per-step scalars.  Column j of slice s is
+// Slices are independent; slice s owns V_s (chunked columns, leading dim ld
+// a multiple of 64, chunks of 2^chunk_shift >= 32 columns whose rows
+// [n, ld) are zero) and per-step scalars.  Column j of slice s is
 //   vtab[s*maxchunks + (j >> chunk_shift)] + (size_t)(j & chunk_mask)*ld
 struct LanBatch {
   int n = 0;

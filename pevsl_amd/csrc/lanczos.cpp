@@ -160,7 +160,8 @@ class BatchSolver {
   void set_active(const std::vector<int> &act);
   void filter_step(const std::vector<int> &act, int k);
   void filter_block(const std::vector<int> &grp, int k);
-  void reorth_and_norm(const std::vector<int> &act, int k);
+  void lanczos_update(const std::vector<int> &act, int k);
+  void cgs(const std::vector<int> &act, int k);
   void breakdown(int s, int k);
   void convergence_test(int s, int k);
   void ritz(int s, double tol);
@@ -184,6 +185,8 @@ class BatchSolver {
   double *pinned_ = nullptr;
   int *pinned_i_ = nullptr;
   std::vector<int> mus_key_;
+  std::vector<double> mus_h_, betas_h_;
+  std::vector<double *> cp_h_;
 };
 
 void BatchSolver::ensure_column(int s, int j) {
@@ -192,7 +195,11 @@ void BatchSolver::ensure_column(int s, int j) {
   bool grew = false;
   while (sl.chunks.size() < need) {
     double *p = nullptr;
-    PV_HIP(hipMalloc(&p, sizeof(double) * L_.ld * ((size_t)1 << L_.chunk_shift)));
+    const size_t bytes = sizeof(double) * L_.ld * ((size_t)1 << L_.chunk_shift);
+    PV_HIP(hipMalloc(&p, bytes));
+    // rows [n, ld) of every column stay zero: the reorthogonalization
+    // kernels stream whole row pairs up to ld
+    PV_HIP(hipMemsetAsync(p, 0, bytes, c_.stream));
     sl.chunks.push_back(p);
     vtab_h_[(size_t)s * L_.maxchunks + sl.chunks.size() - 1] = p;
     grew = true;
@@ -207,6 +214,8 @@ void BatchSolver::sync_vtab() {
 }
 
 void BatchSolver::set_active(const std::vector<int> &act) {
+  // the staging buffer may still feed an earlier copy
+  PV_HIP(hipStreamSynchronize(c_.stream));
   for (int s = 0; s < ns_; s++) pinned_i_[s] = 0;
   for (int s : act) pinned_i_[s] = 1;
   PV_HIP(hipMemcpyAsync(L_.active, pinned_i_, sizeof(int) * ns_,
@@ -246,8 +255,12 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
     key.push_back(s);
   }
   key.push_back(B);
+  // host staging (mus_h_, cp_h_) is rewritten only after the per-step
+  // synchronization, when earlier copies have completed
   if (key != mus_key_) {
-    std::vector<double> mh((size_t)(degmax + 1) * B, 0.0);
+    PV_HIP(hipStreamSynchronize(c_.stream));
+    std::vector<double> &mh = mus_h_;
+    mh.assign((size_t)(degmax + 1) * B, 0.0);
     for (int ci = 0; ci < g; ci++) {
       const pevsl_Polparams *pol = S_[(size_t)grp[(size_t)ci]].pol;
       for (int j = 0; j <= pol->deg; j++) mh[(size_t)j * B + ci] = pol->mu[j];
@@ -258,7 +271,9 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
     mus_key_ = key;
   }
   // column pointers: inputs z_k, outputs z_{k+1}
-  std::vector<double *> cp((size_t)2 * B, nullptr);
+  PV_HIP(hipStreamSynchronize(c_.stream));
+  std::vector<double *> &cp = cp_h_;
+  cp.assign((size_t)2 * B, nullptr);
   for (int ci = 0; ci < g; ci++) {
     cp[(size_t)ci] = col(grp[(size_t)ci], k);
     cp[(size_t)B + ci] = col(grp[(size_t)ci], k + 1);
@@ -332,14 +347,25 @@ void BatchSolver::filter_step(const std::vector<int> &act, int k) {
   }
 }
 
-void BatchSolver::reorth_and_norm(const std::vector<int> &act, int k) {
+// three-term update: alpha and |z_{k+1}|^2 before reorthogonalization
+void BatchSolver::lanczos_update(const std::vector<int> &act, int k) {
   set_active(act);
   lan_alpha(L_, k, c_.stream);
   lan_subtract_alpha(L_, k, c_.stream);
+  const double vecb = 8.0 * n_;
+  c_.kstats.reorth_bytes += act.size() * (4 * vecb + 3 * vecb);
+}
+
+// classical Gram-Schmidt with the DGKS test: pass 0 always, pass 1 only
+// for slices whose norm dropped below 1/sqrt(2) (decided on the device)
+void BatchSolver::cgs(const std::vector<int> &act, int k) {
   lan_cgs_pass(L_, k, 0, c_.stream);
   lan_cgs_pass(L_, k, 1, c_.stream);
   const double vecb = 8.0 * n_;
-  c_.kstats.reorth_bytes += act.size() * (4 * vecb + 3 * vecb + 2.0 * (k + 1) * vecb + 3 * vecb);
+  const double pass = 2.0 * (k + 1) * vecb + 3 * vecb;
+  c_.kstats.reorth_bytes += act.size() * pass;
+  c_.kstats.cgs_bytes += act.size() * pass;
+  c_.kstats.cgs_passes += (long long)act.size();
 }
 
 // Lucky breakdown (SRC/cheblanNr.c:248-281): replace z_{k+1} by a filtered
@@ -437,6 +463,8 @@ void BatchSolver::ritz(int s, double tol) {
       rr((size_t)CH);
   const int csz = 1 << L_.chunk_shift;
   const double one = 1.0, zero = 0.0;
+  const double acc_tol =
+      c_.filter_prec == PEVSL_PREC_FP32 ? std::max(tol, c_.mixed_tol) : tol;
   for (int cs = 0; cs < nsel && sl.rc == 0; cs += CH) {
     const int nb = std::min(CH, nsel - cs);
     // U = V(:, 0:kdim) * C(:, i0+cs : i0+cs+nb), accumulated over basis chunks
@@ -484,7 +512,7 @@ void BatchSolver::ritz(int s, double tol) {
       const double t = tq[(size_t)i];
       if (t < sl.aa - DBL_EPSILON || t > sl.bb + DBL_EPSILON) continue;
       const double r = sqrt(rr[(size_t)i]);
-      if (r < tol) {
+      if (r < acc_tol) {
         sl.lam.push_back(t);
         sl.res.push_back(r);
         if (c_.keep_vectors) {
@@ -563,8 +591,8 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
   // V(:,0) = p(A) vinit / |.|  (FILTER_VINIT, SRC/cheblanNr.c:169-194)
   double *dvi = c_.cb_y.ensure((size_t)n_);
   PV_HIP(hipMemcpyAsync(dvi, vinit->data, sizeof(double) * n_, hipMemcpyHostToDevice, st));
-  EvTimer ev_f, ev_r;
-  double t_filter = 0, t_reorth = 0;
+  EvTimer ev_f, ev_r, ev_c;
+  double t_filter = 0, t_reorth = 0, t_cgs = 0;
   for (int s : act) {
     ensure_column(s, 0);
     cheb_filter_col(c_, S_[(size_t)s].pol, dvi, col(s, 0));
@@ -591,7 +619,10 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
     ev_f.stop(st);
     // ---- three-term update and full reorthogonalization
     ev_r.start(st);
-    reorth_and_norm(act, k);
+    lanczos_update(act, k);
+    ev_c.start(st);
+    cgs(act, k);
+    ev_c.stop(st);
     ev_r.stop(st);
     for (int s = 0; s < ns_; s++) {
       pinned_[s] = 0;
@@ -600,11 +631,22 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
     PV_HIP(hipMemcpyAsync(pinned_, L_.alpha, sizeof(double) * ns_, hipMemcpyDeviceToHost, st));
     PV_HIP(hipMemcpyAsync(pinned_ + ns_, L_.nrm2b, sizeof(double) * ns_,
                           hipMemcpyDeviceToHost, st));
+    PV_HIP(hipMemcpyAsync(pinned_i_, L_.pass2, sizeof(int) * ns_, hipMemcpyDeviceToHost, st));
     PV_HIP(hipStreamSynchronize(st));
     t_filter += ev_f.seconds();
     t_reorth += ev_r.seconds();
+    t_cgs += ev_c.seconds();
+    for (int s : act) {
+      if (pinned_i_[s]) {  // second DGKS pass ran
+        const double pass = 2.0 * (k + 1) * 8.0 * n_ + 3 * 8.0 * n_;
+        c_.kstats.reorth_bytes += pass;
+        c_.kstats.cgs_bytes += pass;
+        c_.kstats.cgs_passes++;
+      }
+    }
     std::vector<int> brk;
-    std::vector<double> betas((size_t)ns_, 0.0);
+    std::vector<double> &betas = betas_h_;
+    betas.assign((size_t)ns_, 0.0);
     for (int s : act) {
       Slice &sl = S_[(size_t)s];
       const double alpha = pinned_[s];
@@ -637,7 +679,10 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
       sl.its = k + 1;
       if ((k < kNtest || (k - kNtest) % kCycle != 0) && k != sl.maxit - 1) continue;
       convergence_test(s, k);
-      if (fstats) fprintf(fstats, "  k %4d:   nconv %4d  tr1 %21.15e\n", k, sl.nconv, sl.tr1);
+      if (fstats) {
+        fprintf(fstats, "  k %4d:   nconv %4d  tr1 %21.15e\n", k, sl.nconv, sl.tr1);
+        fflush(fstats);
+      }
     }
     // retire converged slices: Ritz pairs, then release their basis
     std::vector<int> still;
@@ -661,6 +706,7 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
   p_->stats->t_reorth += t_reorth;
   c_.kstats.t_filter += t_filter;
   c_.kstats.t_reorth += t_reorth;
+  c_.kstats.t_cgs += t_cgs;
   p_->stats->t_iter += wtime() - tall;
   c_.info.assign((size_t)ns_, pevsl_LanInfo{});
   for (int s = 0; s < ns_; s++) {
@@ -748,6 +794,27 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs, int *max
   }
   bs.outputs(nevOut, lamo, Wo, reso, vinit);
   return rc;
+}
+
+int pEVSL_ChebAv(pevsl_Data *pevsl, pevsl_Polparams *pol, pevsl_Parvec *v,
+                 pevsl_Parvec *y, pevsl_Parvec *w) {
+  (void)w;
+  Ctx &c = ctx_of(pevsl);
+  const int n = pevsl->n;
+  if (v->n_local != n || y->n_local != n) return -4;
+  if (!c.A && (int)c.h_x.size() != n) {
+    c.h_x.assign((size_t)n, 0.0);
+    c.h_y.assign((size_t)n, 0.0);
+  }
+  double *d = c.cb_x.ensure((size_t)2 * n);
+  PV_HIP(hipMemcpyAsync(d, v->data, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+  const double t0 = wtime();
+  cheb_filter_col(c, pol, d, d + n);
+  PV_HIP(hipMemcpyAsync(y->data, d + n, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+  PV_HIP(hipStreamSynchronize(c.stream));
+  pevsl->stats->t_polAv += wtime() - t0;
+  pevsl->stats->n_polAv++;
+  return 0;
 }
 
 int pEVSL_ChebLanNr(pevsl_Data *pevsl, double *intv, int maxit, double tol,

@@ -1,0 +1,160 @@
+// tune.cpp -- autotuner of the fused filter step and reorthogonalization
+// timing (pevsl_amd.h: pEVSL_Autotune, pEVSL_BenchReorth).  No counterpart
+// in the reference: it picks the lanes per row of the one-column kernel and
+// the block width of the batched SpMM for the matrix at hand, from HIP-event
+// timings of back-to-back launches on the instance's stream.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.hpp"
+
+using namespace pevsl;
+
+namespace {
+
+double step_bytes(const DevCsr &A, int elem, int ncol) {
+  return (double)A.nnz * (elem + 4) + 4.0 * (A.n + 1) + 5.0 * (double)A.n * elem * ncol;
+}
+
+template <typename T>
+double time_vec(Ctx &c, int lanes, int reps) {
+  const DevCsr &A = *c.A;
+  const int n = A.n;
+  DBuf<T> b;
+  b.ensure((size_t)4 * n);
+  T *v0 = b.p, *v1 = b.p + n, *v2 = b.p + 2 * (size_t)n, *y = b.p + 3 * (size_t)n;
+  PV_HIP(hipMemsetAsync(b.p, 0, sizeof(T) * 4 * (size_t)n, c.stream));
+  // warm-up
+  cheb_step_vec<T>(A, v0, v1, v2, y, 6.0, 1.0 / 6, 0.1, 0.1, false, lanes, c.stream);
+  EvTimer ev;
+  ev.start(c.stream);
+  T *vk = v0, *vkm1 = v1, *vkp1 = v2;
+  for (int r = 0; r < reps; r++) {
+    cheb_step_vec<T>(A, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, 0.1, 0.1, false, lanes, c.stream);
+    T *t = vkm1;
+    vkm1 = vk;
+    vk = vkp1;
+    vkp1 = t;
+  }
+  ev.stop(c.stream);
+  return ev.seconds() / reps;
+}
+
+template <typename T>
+double time_blk(Ctx &c, int B, int reps, DBuf<double> &mus) {
+  const DevCsr &A = *c.A;
+  const size_t blk = (size_t)A.n * B;
+  DBuf<T> b;
+  b.ensure(4 * blk);
+  PV_HIP(hipMemsetAsync(b.p, 0, sizeof(T) * 4 * blk, c.stream));
+  T *v0 = b.p, *v1 = b.p + blk, *v2 = b.p + 2 * blk, *y = b.p + 3 * blk;
+  cheb_step_blk<T>(A, B, v0, v1, v2, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
+  EvTimer ev;
+  ev.start(c.stream);
+  T *vk = v0, *vkm1 = v1, *vkp1 = v2;
+  for (int r = 0; r < reps; r++) {
+    cheb_step_blk<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
+    T *t = vkm1;
+    vkm1 = vk;
+    vk = vkp1;
+    vkp1 = t;
+  }
+  ev.stop(c.stream);
+  return ev.seconds() / reps;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pEVSL_Autotune(pevsl_Data *p, int reps, pevsl_TuneResult *res) {
+  Ctx &c = ctx_of(p);
+  if (!c.A) return -1;
+  if (reps < 1) reps = 1;
+  const bool f32 = c.filter_prec == PEVSL_PREC_FP32;
+  const int elem = f32 ? 4 : 8;
+  *res = pevsl_TuneResult{};
+  double best = 1e30;
+  int bl = 1;
+  for (int i = 0; i < 6; i++) {
+    const int L = 1 << i;
+    const double t = f32 ? time_vec<float>(c, L, reps) : time_vec<double>(c, L, reps);
+    res->t_lanes[i] = t;
+    if (t < best) {
+      best = t;
+      bl = L;
+    }
+  }
+  res->bytes_lanes = step_bytes(*c.A, elem, 1);
+  DBuf<double> mus;
+  mus.ensure(3 * 64);
+  std::vector<double> mh(3 * 64, 0.01);
+  PV_HIP(hipMemcpy(mus.p, mh.data(), sizeof(double) * mh.size(), hipMemcpyHostToDevice));
+  double bestc = 1e30;
+  int bw = 1;
+  for (int i = 0; i < 6; i++) {
+    const int B = 2 << i;
+    // skip widths whose blocks would not fit comfortably (4 blocks)
+    const double need = 4.0 * c.A->n * (double)B * elem;
+    if (need > 32e9) break;
+    const double t = f32 ? time_blk<float>(c, B, reps, mus) : time_blk<double>(c, B, reps, mus);
+    res->t_block[i] = t;
+    res->bytes_block[i] = step_bytes(*c.A, elem, B);
+    if (t / B < bestc) {
+      bestc = t / B;
+      bw = B;
+    }
+  }
+  res->lanes = bl;
+  res->block_width = bw;
+  c.lanes = bl;
+  c.block_width = bw;
+  return 0;
+}
+
+int pEVSL_BenchReorth(pevsl_Data *p, int k, int reps, double *sec, double *bytes) {
+  Ctx &c = ctx_of(p);
+  const int n = p->n;
+  if (n <= 0 || k < 1) return -1;
+  LanBatch L;
+  L.n = n;
+  L.ld = ((size_t)n + 63) / 64 * 64;
+  L.nslice = 1;
+  int shift = 0;
+  while ((1 << shift) < k + 1 || shift < 5) shift++;
+  L.chunk_shift = shift;
+  L.maxchunks = 1;
+  DBuf<double> V, sc, h, part;
+  DBuf<double *> vt;
+  DBuf<int> fl;
+  V.ensure(L.ld * ((size_t)1 << shift));
+  rand_fill(L.ld * ((size_t)1 << shift), V.p, 7, 0, 0, c.stream);
+  L.vtab = vt.ensure(1);
+  PV_HIP(hipMemcpy(L.vtab, &V.p, sizeof(double *), hipMemcpyHostToDevice));
+  double *s = sc.ensure(4);
+  L.alpha = s;
+  L.beta = s + 1;
+  L.nrm2a = s + 2;
+  L.nrm2b = s + 3;
+  int *f = fl.ensure(2);
+  L.pass2 = f;
+  L.active = f + 1;
+  const int one[2] = {0, 1};
+  PV_HIP(hipMemcpy(f, one, sizeof(int) * 2, hipMemcpyHostToDevice));
+  PV_HIP(hipMemset(s, 0, sizeof(double) * 4));
+  L.hcap = k + 1;
+  L.h = h.ensure((size_t)L.hcap);
+  L.part_cap = lan_part_doubles(n, 1, L.hcap);
+  L.part = part.ensure(L.part_cap);
+  const int kk = k - 1;  // basis columns 0..k-1, vector in column k
+  lan_cgs_pass(L, kk, 0, c.stream);
+  EvTimer ev;
+  ev.start(c.stream);
+  for (int r = 0; r < reps; r++) lan_cgs_pass(L, kk, 0, c.stream);
+  ev.stop(c.stream);
+  *sec = ev.seconds() / reps;
+  *bytes = 2.0 * 8.0 * (double)n * k + 4.0 * 8.0 * n;
+  return 0;
+}
+
+}  // extern "C"

@@ -208,7 +208,14 @@ def rand_vinit(lib, n):
     return v
 
 
-def cheblan_batch(inst, intvs, maxits, tol, vinit, pols, keep_vectors=False):
+def c_stdout():
+    """libc's stdout FILE* (for the fstats argument)."""
+    libc = C.CDLL(None)
+    return C.c_void_p.in_dll(libc, "stdout")
+
+
+def cheblan_batch(inst, intvs, maxits, tol, vinit, pols, keep_vectors=False,
+                  verbose=False):
     """pEVSL_ChebLanNrBatch over len(pols) slices.  Returns a list of dicts
     with lam (unsorted, reference order), res, vecs (or None) and info."""
     lib = inst.lib
@@ -228,7 +235,7 @@ def cheblan_batch(inst, intvs, maxits, tol, vinit, pols, keep_vectors=False):
     rc = lib.pEVSL_ChebLanNrBatch(inst.p, ns, iv.ctypes.data_as(capi.c_double_p),
                                   mi.ctypes.data_as(capi.c_int_p), tol, C.byref(pv),
                                   parr, nev.ctypes.data_as(capi.c_int_p), lamo, Wo,
-                                  reso, None)
+                                  reso, c_stdout() if verbose else None)
     libc = C.CDLL(None)
     libc.free.argtypes = [C.c_void_p]
     out = []
@@ -254,7 +261,8 @@ def cheblan_batch(inst, intvs, maxits, tol, vinit, pols, keep_vectors=False):
 def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
                     prec=capi.PEVSL_PREC_FP64, batch=False, slices_only=None,
                     keep_vectors=False, rand_mode=capi.PEVSL_RAND_LIBC, seed=0,
-                    sli_override=None, block_width=0, log=None):
+                    sli_override=None, block_width=0, log=None, verbose=False,
+                    maxit_cap=None):
     """End-to-end run of the TESTS/Lap/LapPLanN.c flow on the GPU.  Returns a
     dict with the DOS, slices, per-slice filters, eigenvalues and timings."""
     lib = capi.load()
@@ -293,6 +301,8 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
         todo = list(range(nslices)) if slices_only is None else list(slices_only)
         nev = ev_int + 2
         mlan = min(max(5 * nev, 300), n)
+        if maxit_cap:
+            mlan = min(mlan, maxit_cap)
         pols, intvs = [], []
         t2 = time.time()
         for sl in todo:
@@ -306,7 +316,8 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
         groups = [list(range(len(todo)))] if batch else [[i] for i in range(len(todo))]
         for g in groups:
             rc, outs = cheblan_batch(inst, [intvs[i] for i in g], [mlan] * len(g), tol,
-                                     vinit, [pols[i] for i in g], keep_vectors)
+                                     vinit, [pols[i] for i in g], keep_vectors,
+                                     verbose)
             if rc:
                 raise RuntimeError("ChebLanNr error %d" % rc)
             for i, o in zip(g, outs):

@@ -1,0 +1,196 @@
+"""CPU tests of libpevsl_amd.so: it loads, exports every symbol the headers
+declare, and its host-side logic (filter design, slicers, sparse/vector
+utilities) agrees with the oracle.  No GPU call is made here."""
+import ctypes as C
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from pevsl_amd import capi, driver
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_functions():
+    names = set()
+    for h in ("pevsl.h", "pevsl_amd.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^\s*(?:int|void|double)\s+\*?\s*(\w+)\s*\(", txt, re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert len(names) > 80
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the binding covers exactly the declared set
+    assert names == set(capi.SIGNATURES), names ^ set(capi.SIGNATURES)
+
+
+def test_struct_layouts_match_headers():
+    # sizes of the public structs on x86-64 (field order of pevsl_struct.h)
+    assert C.sizeof(capi.Csr) == 32
+    assert C.sizeof(capi.Parvec) == 24
+    assert C.sizeof(capi.Parvecs) == 32
+    assert C.sizeof(capi.Polparams) == 96
+    assert C.sizeof(capi.Stat) == 17 * 8 + 10 * 8
+    assert capi.Polparams.deg.offset == 88
+
+
+def pol_from_lib(lib, intv, **kw):
+    pol, rc = driver.find_pol(lib, intv, **kw)
+    return pol, rc
+
+
+@pytest.mark.parametrize("intv", [[0.6, 0.7431715857928962, 0, 12],
+                                  [1.098549274637319, 1.2, 0, 12],
+                                  [0.0, 0.3, 0, 12], [11.6, 12.0, 0, 12],
+                                  [0.6, 1.2, 0, 12], [5.0, 5.001, 0, 12],
+                                  [-1.0, 0.2, 0, 12]])
+def test_findpol_matches_oracle(lib, intv):
+    pol, rc = pol_from_lib(lib, intv)
+    opol = orc.Pol(intv)
+    assert rc == opol.rc == 0
+    assert pol.deg == opol.deg and pol.type == opol.type
+    assert abs(pol.bar - opol.bar) <= 1e-14 * max(1, abs(opol.bar))
+    assert abs(pol.gam - opol.gam) <= 1e-14
+    np.testing.assert_allclose(driver.pol_mu(pol), opol.mu, rtol=1e-13, atol=1e-16)
+    lib.pEVSL_FreePol(C.byref(pol))
+
+
+def test_findpol_readme_degrees(lib):
+    g = json.load(open(os.path.join(GOLD, "readme_c1.json")))
+    sli = g["sli"]
+    for s in g["slices"]:
+        pol, rc = pol_from_lib(lib, [sli[s["slice"]], sli[s["slice"] + 1], 0, 12])
+        assert rc == 0 and pol.deg == s["deg"]
+        lib.pEVSL_FreePol(C.byref(pol))
+
+
+def test_findpol_forced_degree_and_invalid(lib):
+    pol, rc = pol_from_lib(lib, [0.6, 1.2, 0, 12], deg=20)
+    opol = orc.Pol([0.6, 1.2, 0, 12], deg=20)
+    assert rc == 0 and pol.deg == opol.deg
+    lib.pEVSL_FreePol(C.byref(pol))
+    _, rc = pol_from_lib(lib, [1.2, 0.6, 0, 12])
+    assert rc == -1
+
+
+def test_spslicer_lan_matches_oracle(lib):
+    x = np.linspace(0.6, 1.2, 200)
+    y = 1.0 + np.sin(7 * x) ** 2
+    sli = driver.spslicer_lan(lib, x, y, 6)
+    ref = orc.spslicer_lan(x, y, 6)
+    assert np.array_equal(sli, ref)
+
+
+def test_linspace_sort_part1d_bsearch(lib):
+    a = np.zeros(7)
+    lib.linspace(0.1, 0.7, 7, a.ctypes.data_as(capi.c_double_p))
+    assert a[0] == 0.1 and a[-1] == 0.7 and np.allclose(np.diff(a), 0.1)
+    v = np.array([3.0, -1.0, 2.0, 2.0, 0.5])
+    ind = np.zeros(5, dtype=np.int32)
+    vv = v.copy()
+    lib.sort_double(5, vv.ctypes.data_as(capi.c_double_p), ind.ctypes.data_as(capi.c_int_p))
+    assert list(vv) == sorted(v) and list(v[ind]) == sorted(v)
+    # 1-D partitions (SRC/utils.c:121-124): 9 -> [3,2,2,2], 10 -> [3,3,2,2]
+    for ln, sizes in ((9, [3, 2, 2, 2]), (10, [3, 3, 2, 2]), (11, [3, 3, 3, 2])):
+        got = []
+        for p in range(4):
+            idx, j1, j2 = C.c_int(p), C.c_int(), C.c_int()
+            lib.pEVSL_Part1d(ln, 4, C.byref(idx), C.byref(j1), C.byref(j2), 1)
+            got.append(j2.value - j1.value)
+        assert got == sizes
+    x = np.array([1, 4, 9, 16], dtype=np.int32)
+    assert lib.pEVSL_BinarySearch(x.ctypes.data_as(capi.c_int_p), 4, 9) == 2
+    assert lib.pEVSL_BinarySearch(x.ctypes.data_as(capi.c_int_p), 4, 5) == -1
+
+
+def test_host_csr_matvec_and_vectors(lib):
+    ia, ja, a = driver.laplacian3d(8, 7, 6)
+    n = len(ia) - 1
+    csr = capi.Csr(n, n, ia.ctypes.data_as(capi.c_int_p), ja.ctypes.data_as(capi.c_int_p),
+                   a.ctypes.data_as(capi.c_double_p))
+    x = np.sin(np.arange(n, dtype=np.float64))
+    y = np.zeros(n)
+    lib.pEVSL_Matvec(C.byref(csr), x.ctypes.data_as(capi.c_double_p),
+                     y.ctypes.data_as(capi.c_double_p))
+    ref = orc.Csr(ia, ja, a).matvec(x)
+    assert np.array_equal(y, ref)
+    # Parvec utilities
+    X, Y = capi.Parvec(), capi.Parvec()
+    yy = np.cos(np.arange(n, dtype=np.float64))
+    lib.pEVSL_ParvecCreateShell(n, n, 0, 0, C.byref(X), x.ctypes.data_as(capi.c_double_p))
+    lib.pEVSL_ParvecCreateShell(n, n, 0, 0, C.byref(Y), yy.ctypes.data_as(capi.c_double_p))
+    t = C.c_double()
+    lib.pEVSL_ParvecDot(C.byref(X), C.byref(Y), C.byref(t))
+    assert abs(t.value - x @ np.cos(np.arange(n))) < 1e-10
+    lib.pEVSL_ParvecAxpy(2.0, C.byref(X), C.byref(Y))
+    np.testing.assert_allclose(yy, np.cos(np.arange(n)) + 2 * x)
+    assert lib.pEVSL_ParvecSameSize(C.byref(X), C.byref(Y)) == 1
+
+
+def test_parvecs_gemv_gemtv(lib):
+    rng = np.random.default_rng(0)
+    n, k, ld = 50, 6, 53
+    data = rng.normal(size=(k, ld))
+    Vs = capi.Parvecs()
+    lib.pEVSL_ParvecsCreateShell(n, k, ld, n, 0, 0, C.byref(Vs),
+                                 data.ctypes.data_as(capi.c_double_p))
+    x = rng.normal(size=k)
+    y = rng.normal(size=n)
+    y0 = y.copy()
+    Y = capi.Parvec()
+    lib.pEVSL_ParvecCreateShell(n, n, 0, 0, C.byref(Y), y.ctypes.data_as(capi.c_double_p))
+    lib.pEVSL_ParvecsGemv(-1.5, C.byref(Vs), k, x.ctypes.data_as(capi.c_double_p), 0.5,
+                          C.byref(Y))
+    V = data[:, :n].T
+    np.testing.assert_allclose(y, 0.5 * y0 - 1.5 * V @ x, atol=1e-12)
+    out = np.ones(k)
+    lib.pEVSL_ParvecsGemtv(2.0, C.byref(Vs), k, C.byref(Y), 1.0,
+                           out.ctypes.data_as(capi.c_double_p))
+    np.testing.assert_allclose(out, 1.0 + 2.0 * V.T @ y, atol=1e-10)
+
+
+def test_coo_csr_roundtrip_and_sortrow(lib):
+    class Coo(C.Structure):
+        _fields_ = [("nrows", C.c_int), ("ncols", C.c_int), ("nnz", C.c_int),
+                    ("ir", capi.c_int_p), ("jc", capi.c_int_p), ("vv", capi.c_double_p)]
+    rng = np.random.default_rng(1)
+    nnz, n = 40, 9
+    ir = rng.integers(0, n, nnz).astype(np.int32)
+    jc = rng.integers(0, n, nnz).astype(np.int32)
+    vv = rng.normal(size=nnz)
+    coo = Coo(n, n, nnz, ir.ctypes.data_as(capi.c_int_p), jc.ctypes.data_as(capi.c_int_p),
+              vv.ctypes.data_as(capi.c_double_p))
+    csr = capi.Csr()
+    assert lib.pEVSL_CooToCsr(0, C.byref(coo), C.byref(csr)) == 0
+    ia = np.ctypeslib.as_array(csr.ia, shape=(n + 1,)).copy()
+    assert ia[-1] == nnz and np.all(np.diff(ia) >= 0)
+    dense = np.zeros((n, n))
+    np.add.at(dense, (ir, jc), vv)
+    x = rng.normal(size=n)
+    y = np.zeros(n)
+    lib.pEVSL_Matvec(C.byref(csr), x.ctypes.data_as(capi.c_double_p),
+                     y.ctypes.data_as(capi.c_double_p))
+    np.testing.assert_allclose(y, dense @ x, atol=1e-12)
+    lib.pEVSL_SortRow(C.byref(csr))
+    ja = np.ctypeslib.as_array(csr.ja, shape=(nnz,))
+    for r in range(n):
+        assert np.all(np.diff(ja[ia[r]:ia[r + 1]]) >= 0)
+    lib.pEVSL_FreeCsr(C.byref(csr))
+
+
+def test_generalized_setters_report_not_implemented(lib):
+    assert lib.pEVSL_SetGenEig(None) == capi.PEVSL_NOT_IMPLEMENT
+    assert lib.pEVSL_SetBSol(None, None, None) == capi.PEVSL_NOT_IMPLEMENT
+    assert lib.pEVSL_LanTrbounds(None, 10, 10, 1e-8, None, 0, None, None, None) == \
+        capi.PEVSL_NOT_IMPLEMENT

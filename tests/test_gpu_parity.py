@@ -1,0 +1,284 @@
+"""GPU parity tests: the HIP path through the C-ABI against the oracle on
+the same inputs (same libc rand() stream), and against the closed-form
+Laplacian spectrum.  Tolerances are written per test:
+  * sparse products / filter with one lane per row: bitwise (same rounding
+    sequence as the reference's scalar loop); with several lanes per row the
+    row sum is a tree: relative 1e-14;
+  * KPM moments: relative 1e-12 (reduction order of the dots differs);
+  * eigenvalues (FP64 path): relative 1e-10 or better against the closed
+    form, and the same count per slice as the oracle's ChebLanNr.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from pevsl_amd import capi, driver
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print("\n[gpu case] " + msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def lap20(lib):
+    ia, ja, a = driver.laplacian3d(20, 20, 20)
+    M = driver.Matrix(lib, ia, ja, a)
+    yield M, orc.Csr(ia, ja, a)
+    M.free()
+
+
+def parvec(lib, x):
+    v = capi.Parvec()
+    lib.pEVSL_ParvecCreateShell(len(x), len(x), 0, 0, C.byref(v),
+                                x.ctypes.data_as(capi.c_double_p))
+    return v
+
+
+@pytest.mark.parametrize("lanes", [0, 1, 2, 4, 8])
+def test_device_matvec_sin_recipe(lib, lap20, lanes):
+    banner("ParcsrMatvec 20^3 x=sin(i) lanes=%d" % lanes)
+    M, A = lap20
+    x = np.sin(np.arange(M.n, dtype=np.float64))
+    if lanes == 0:
+        y = M.matvec(x)  # pEVSL_ParcsrMatvec, host vectors
+    else:
+        inst = driver.Instance(lib, M, lanes=lanes)
+        hip = C.CDLL("libamdhip64.so")
+        d = C.c_void_p()
+        nb = x.nbytes
+        assert hip.hipMalloc(C.byref(d), C.c_size_t(2 * nb)) == 0
+        try:
+            dx, dy = d.value, d.value + nb
+            assert hip.hipMemcpy(C.c_void_p(dx), x.ctypes.data_as(C.c_void_p),
+                                 C.c_size_t(nb), 1) == 0  # host to device
+            assert lib.pEVSL_DeviceMatvec(inst.p, C.c_void_p(dx), C.c_void_p(dy)) == 0
+            y = np.empty_like(x)
+            assert hip.hipMemcpy(y.ctypes.data_as(C.c_void_p), C.c_void_p(dy),
+                                 C.c_size_t(nb), 2) == 0  # device to host
+        finally:
+            hip.hipFree(d)
+            inst.close()
+    ref = A.matvec(x)
+    if lanes == 1:
+        assert np.array_equal(y, ref)
+    else:
+        assert np.max(np.abs(y - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("lanes,prec", [(1, 0), (2, 0), (8, 0), (2, 1)])
+def test_cheb_av_against_oracle(lib, lap20, lanes, prec):
+    banner("ChebAv 20^3 lanes=%d prec=%d" % (lanes, prec))
+    M, A = lap20
+    iv = [0.87, 0.99, 0.0, 12.0]
+    pol, rc = driver.find_pol(lib, iv)
+    opol = orc.Pol(iv)
+    assert rc == 0 and pol.deg == opol.deg
+    inst = driver.Instance(lib, M, prec=prec, lanes=lanes)
+    try:
+        v = np.random.default_rng(0).uniform(-1, 1, M.n)
+        y = np.zeros(M.n)
+        rc = lib.pEVSL_ChebAv(inst.p, C.byref(pol), C.byref(parvec(lib, v)),
+                              C.byref(parvec(lib, y)), None)
+        assert rc == 0
+    finally:
+        inst.close()
+        lib.pEVSL_FreePol(C.byref(pol))
+    ref = orc.cheb_av(A, opol, v)
+    scale = np.max(np.abs(ref))
+    if prec == 1:
+        # FP32 filter: error relative to the filtered vector's size
+        assert np.max(np.abs(y - ref)) <= 1e-4 * scale
+    elif lanes == 1:
+        assert np.array_equal(y, ref)
+    else:
+        assert np.max(np.abs(y - ref)) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_kpmdos_and_slicer_against_oracle(lib, lap20, prec):
+    banner("Kpmdos 20^3 Mdeg=300 nvec=60 prec=%d + SpslicerKpm" % prec)
+    M, A = lap20
+    intv = [0.6, 1.2, 0.0, 12.0]
+    inst = driver.Instance(lib, M, prec=prec)  # pEVSL_Start seeds srand(0)
+    try:
+        rc, mu, ecnt = inst.kpmdos(300, 1, 60, intv)
+    finally:
+        inst.close()
+    assert rc == 0
+    orc.srand(0)
+    rc, omu, oecnt = orc.kpmdos(A, 300, 1, 60, intv)
+    if prec == 0:
+        np.testing.assert_allclose(mu, omu, rtol=1e-12, atol=1e-15)
+        assert abs(ecnt - oecnt) < 1e-9
+        rc, sli = driver.spslicer_kpm(lib, mu, intv, 5, int(10 * ecnt))
+        orc_rc, osli = orc.spslicer_kpm(omu, intv, 5, int(10 * oecnt))
+        assert rc == orc_rc == 0
+        np.testing.assert_allclose(sli, osli, rtol=0, atol=1e-12)
+    else:
+        np.testing.assert_allclose(mu, omu, rtol=0, atol=2e-5)
+        assert abs(ecnt - oecnt) < 0.05
+
+
+def test_spslicer_device_integral_matches_oracle(lib):
+    banner("SpslicerKpm random moments, 5000 points")
+    rng = np.random.default_rng(3)
+    mu = rng.normal(size=301) * np.exp(-np.arange(301) / 50.0)
+    mu[0] = abs(mu[0]) + 1.0
+    intv = [0.6, 1.2, 0.0, 12.0]
+    rc, sli = driver.spslicer_kpm(lib, mu, intv, 7, 5000)
+    orc_rc, osli = orc.spslicer_kpm(mu, intv, 7, 5000)
+    assert rc == orc_rc
+    np.testing.assert_allclose(sli, osli, atol=1e-12)
+
+
+def test_cheblan_single_slice_matches_oracle(lib):
+    banner("ChebLanNr 12^3 [0.6,1.2] FP64 vs oracle")
+    nx = 12
+    ia, ja, a = driver.laplacian3d(nx, nx, nx)
+    M = driver.Matrix(lib, ia, ja, a)
+    inst = driver.Instance(lib, M)
+    intv = [0.6, 1.2, 0.0, 12.0]
+    try:
+        pol, rc = driver.find_pol(lib, intv)
+        vinit = driver.rand_vinit(lib, M.n)
+        rc, outs = driver.cheblan_batch(inst, [intv], [300], 1e-8, vinit, [pol],
+                                        keep_vectors=True)
+        lib.pEVSL_FreePol(C.byref(pol))
+    finally:
+        inst.close()
+    assert rc == 0
+    o = outs[0]
+    A = orc.Csr(ia, ja, a)
+    ref = orc.cheblan_nr(A, intv, 300, 1e-8, vinit, orc.Pol(intv))
+    ex = orc.exact_eig_lap3(nx, nx, nx, 0.6, 1.2)
+    assert len(o["lam"]) == len(ref["lam"]) == len(ex)
+    np.testing.assert_allclose(np.sort(o["lam"]), ex, rtol=1e-10)
+    np.testing.assert_allclose(np.sort(o["lam"]), np.sort(ref["lam"]), rtol=1e-12)
+    assert o["info"]["its"] == ref["its"] and o["info"]["nbreak"] == ref["nbreak"]
+    assert np.all(o["res"] < 1e-8)
+    # returned Ritz vectors: unit norm, residual as reported
+    for lam, y, r in zip(o["lam"], o["vecs"], o["res"]):
+        assert abs(np.linalg.norm(y) - 1.0) < 1e-12
+        assert abs(np.linalg.norm(A.matvec(y) - lam * y) - r) < 1e-12
+    M.free()
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_c1_end_to_end_counts(lib, batch):
+    banner("C1 20^3 [0.6,1.2] 5 slices batch=%s" % batch)
+    r = driver.lap_slicing_run(20, 20, 20, 0.6, 1.2, 5, batch=batch)
+    d = orc.run_driver(["-nx", 20, "-ny", 20, "-nz", 20, "-a", 0.6, "-b", 1.2,
+                        "-nslices", 5], timeout=120)
+    assert abs(r["ecount"] - d["ecount"]) < 1e-9
+    np.testing.assert_allclose(r["sli"], d["sli"], atol=1e-12)
+    total = 0
+    for s, ds in zip(r["slices"], d["slices"]):
+        assert s["deg"] == ds["deg"]
+        assert s["nev"] == s["nev_exact"] == ds["nev"]
+        assert s["max_rel_err"] < 1e-10
+        assert np.all(s["res"] < 1e-8)
+        total += s["nev"]
+    assert total == 118
+
+
+def test_c1_readme_slices(lib):
+    banner("README slices 20^3: counts 18/27/27/19/27, degrees 61..110")
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "readme_c1.json")))
+    r = driver.lap_slicing_run(20, 20, 20, 0.6, 1.2, 5, batch=True,
+                               sli_override=g["sli"])
+    for s, gs in zip(r["slices"], g["slices"]):
+        assert s["deg"] == gs["deg"]
+        assert s["nev"] == gs["found"] == gs["exact"]
+
+
+def test_batch_equals_single_slices(lib):
+    banner("batched vs one-by-one solve, 16^3, 4 slices")
+    r1 = driver.lap_slicing_run(16, 16, 16, 0.4, 1.0, 4, batch=False)
+    r2 = driver.lap_slicing_run(16, 16, 16, 0.4, 1.0, 4, batch=True)
+    for a, b in zip(r1["slices"], r2["slices"]):
+        assert a["nev"] == b["nev"] == a["nev_exact"]
+        np.testing.assert_allclose(a["lam"], b["lam"], rtol=1e-12)
+
+
+def test_lanczos_dos_against_oracle(lib):
+    banner("LanDosG 10^3 nvec=12 msteps=40 vs oracle")
+    ia, ja, a = driver.laplacian3d(10, 10, 10)
+    M = driver.Matrix(lib, ia, ja, a)
+    intv = [0.6, 1.2, 0.0, 12.0]
+    inst = driver.Instance(lib, M)
+    try:
+        rc, xd, yd, neig = inst.landos(12, 40, 200, intv)
+    finally:
+        inst.close()
+        M.free()
+    assert rc == 0
+    orc.srand(0)
+    orc_rc, oxd, oyd, oneig = orc.landos(orc.Csr(ia, ja, a), 12, 40, 200, intv)
+    assert np.array_equal(xd, oxd)
+    np.testing.assert_allclose(yd, oyd, rtol=1e-8, atol=1e-10)
+    assert abs(neig - oneig) < 1e-6 * max(1.0, abs(oneig))
+    sli = driver.spslicer_lan(lib, xd, yd, 4)
+    assert np.all(np.diff(sli) > 0)
+
+
+def test_host_matvec_callback_path(lib):
+    banner("SetAMatvec host callback, 8^3 slice")
+    ia, ja, a = driver.laplacian3d(8, 8, 8)
+    A = orc.Csr(ia, ja, a)
+    n = A.n
+    CB = C.CFUNCTYPE(None, capi.c_double_p, capi.c_double_p, C.c_void_p)
+
+    def mv(x, y, data):
+        xv = np.ctypeslib.as_array(x, shape=(n,))
+        yv = np.ctypeslib.as_array(y, shape=(n,))
+        yv[:] = A.matvec(xv.copy())
+    cb = CB(mv)
+    inst = driver.Instance(lib)
+    try:
+        lib.pEVSL_SetProbSizes(inst.p, n, n, 0)
+        lib.pEVSL_SetAMatvec(inst.p, C.cast(cb, C.c_void_p), None)
+        intv = [0.6, 1.8, 0.0, 12.0]
+        pol, rc = driver.find_pol(lib, intv)
+        vinit = driver.rand_vinit(lib, n)
+        rc, outs = driver.cheblan_batch(inst, [intv], [200], 1e-8, vinit, [pol])
+        lib.pEVSL_FreePol(C.byref(pol))
+    finally:
+        inst.close()
+    assert rc == 0
+    ex = orc.exact_eig_lap3(8, 8, 8, 0.6, 1.8)
+    assert len(outs[0]["lam"]) == len(ex)
+    np.testing.assert_allclose(np.sort(outs[0]["lam"]), ex, rtol=1e-10)
+
+
+def test_invalid_interval_returns_nothing(lib):
+    banner("ChebLanNr invalid interval")
+    ia, ja, a = driver.laplacian3d(6, 6, 6)
+    M = driver.Matrix(lib, ia, ja, a)
+    inst = driver.Instance(lib, M)
+    try:
+        pol, _ = driver.find_pol(lib, [0.6, 1.2, 0, 12])
+        vinit = driver.rand_vinit(lib, M.n)
+        rc, outs = driver.cheblan_batch(inst, [[1.2, 0.6, 0, 12]], [100], 1e-8, vinit, [pol])
+        assert rc == 0 and len(outs[0]["lam"]) == 0
+        lib.pEVSL_FreePol(C.byref(pol))
+    finally:
+        inst.close()
+        M.free()
+
+
+def test_reorth_kernel_orthogonality(lib):
+    banner("40^3 slice with ~90 eigenvalues: counts, orthogonality")
+    r = driver.lap_slicing_run(40, 40, 40, 0.6, 1.2, 8, slices_only=[3], batch=True,
+                               keep_vectors=True)
+    s = r["slices"][0]
+    assert s["nev"] == s["nev_exact"]
+    assert s["max_rel_err"] < 1e-10
+    Y = s["vecs"]
+    G = Y @ Y.T
+    assert np.max(np.abs(G - np.eye(len(Y)))) < 1e-8
