@@ -21,6 +21,9 @@
 /* relative threshold of the lucky-breakdown test (INC/pevsl_def.h:18) */
 #define orthTol 1e-14
 
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
 #ifndef PI
 #define PI M_PI
 #endif
