@@ -61,6 +61,14 @@ int pEVSL_SetFilterPrecision(pevsl_Data *pevsl, int prec);
  * residuals use this threshold; eigenvalue errors are reported separately. */
 int pEVSL_SetMixedTol(pevsl_Data *pevsl, double tol);
 
+/* Ritz stage: 0 = the reference's per-vector Ritz pairs (SRC/cheblanNr.c:
+ * 339-413); 1 = FP64 Rayleigh-Ritz refinement of the selected Ritz vectors
+ * with A ((U^T A U) c = theta (U^T U) c, DGEMM Gram products + SYGVD) before
+ * the same acceptance test; -1 (default) = refinement only with the FP32
+ * filter.  The refinement separates eigenvectors that the filter resolves
+ * only to its own precision. */
+int pEVSL_SetRitzRefine(pevsl_Data *pevsl, int mode);
+
 /* Random vectors: PEVSL_RAND_LIBC draws (rand()-RAND_MAX/2)/(RAND_MAX/2) from
  * libc on the host in the reference order (SRC/parvec.c:65-72);
  * PEVSL_RAND_RADEMACHER / PEVSL_RAND_UNIFORM draw +-1 / uniform[-1,1] from a
@@ -73,10 +81,20 @@ int pEVSL_SetRandMode(pevsl_Data *pevsl, int mode, unsigned long long seed);
 /* Whether ChebLanNr copies the Ritz vectors back (*Wo); 1 by default. */
 int pEVSL_SetKeepVectors(pevsl_Data *pevsl, int keep);
 
-/* SpMV lanes per row (1..64, power of two; 0 = by mean row length) and the
- * block width of batched solves (1..64, power of two; 0 = number of slices
- * rounded up). */
+/* Kernel of the one-column filter step: lanes per row of the vector-CSR
+ * kernel (1..64, power of two), PEVSL_SPMV_STREAM for the CSR-stream kernel
+ * (row blocks staged in LDS), or 0 = automatic; and the block width of
+ * batched solves (1..64, power of two; 0 = number of slices rounded up). */
+#define PEVSL_SPMV_STREAM 128
 int pEVSL_SetSpmvTuning(pevsl_Data *pevsl, int lanes_per_row, int block_width);
+
+/* Eigenvalues >= thr of the symmetric tridiagonal matrix (diag[0..n-1],
+ * sdiag[0..n-2]) by Sturm multisection on the GPU (thr = -HUGE_VAL: all), in
+ * ascending order in eigval[0..*count-1], and their sum in *trace.  The
+ * restricted-trace test of ChebLanNr uses the same kernels
+ * (SRC/cheblanNr.c:298-321 computes it with LAPACK DSTEV). */
+int pEVSL_SymmTridEigvals(int n, const double *diag, const double *sdiag, double thr,
+                          double *eigval, int *count, double *trace);
 
 /* Batched filtered Lanczos: nslices slices [intvs[4*s], intvs[4*s+1]] (with
  * bounds intvs[4*s+2..3]) advance together, one column of a block SpMM each,
@@ -110,19 +128,21 @@ int pEVSL_DeviceInfo(char *name, int len, size_t *hbm_bytes);
 /* Autotuner of the fused filter step on the instance's matrix.  Times
  * `reps` back-to-back Chebyshev steps (HIP events) for every candidate and
  * records the fastest:
- *   one column: lanes per row in {1,2,4,8,16,32}            -> lanes
+ *   one column: lanes per row in {1,2,4,8,16,32} or the CSR-stream
+ *               kernel                                       -> lanes
  *   block SpMM: widths {2,4,8,16,32,64}, per-column cost     -> block_width
  * for the instance's filter precision, and installs the choice (as
  * pEVSL_SetSpmvTuning).  Results: seconds per step of each candidate
  * (t_lanes[i] for lanes 2^i, t_block[i] for width 2^(i+1)) and the
  * algorithmic bytes of one step (bytes_lanes, bytes_block[i]). */
 typedef struct {
-  int lanes;
+  int lanes;          /* chosen: lanes per row, or PEVSL_SPMV_STREAM */
   int block_width;
   double t_lanes[6];
   double bytes_lanes;
   double t_block[6];
   double bytes_block[6];
+  double t_stream;    /* CSR-stream kernel, seconds per step */
 } pevsl_TuneResult;
 int pEVSL_Autotune(pevsl_Data *pevsl, int reps, pevsl_TuneResult *res);
 

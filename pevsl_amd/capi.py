@@ -17,6 +17,7 @@ PEVSL_PREC_FP32 = 1
 PEVSL_RAND_LIBC = 0
 PEVSL_RAND_RADEMACHER = 1
 PEVSL_RAND_UNIFORM = 2
+PEVSL_SPMV_STREAM = 128
 
 c_double_p = C.POINTER(C.c_double)
 c_int_p = C.POINTER(C.c_int)
@@ -84,7 +85,8 @@ class LanInfo(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("lanes", C.c_int), ("block_width", C.c_int),
                 ("t_lanes", C.c_double * 6), ("bytes_lanes", C.c_double),
-                ("t_block", C.c_double * 6), ("bytes_block", C.c_double * 6)]
+                ("t_block", C.c_double * 6), ("bytes_block", C.c_double * 6),
+                ("t_stream", C.c_double)]
 
 
 class KernelStats(C.Structure):
@@ -209,6 +211,7 @@ SIGNATURES = {
     # pevsl_amd.h extensions
     "pEVSL_SetFilterPrecision": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetMixedTol": (C.c_int, [DataP, C.c_double]),
+    "pEVSL_SetRitzRefine": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetRandMode": (C.c_int, [DataP, C.c_int, C.c_ulonglong]),
     "pEVSL_SetKeepVectors": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetSpmvTuning": (C.c_int, [DataP, C.c_int, C.c_int]),
@@ -224,6 +227,8 @@ SIGNATURES = {
     "pEVSL_GetKernelStats": (C.c_int, [DataP, C.POINTER(KernelStats)]),
     "pEVSL_DeviceMatvec": (C.c_int, [DataP, C.c_void_p, C.c_void_p]),
     "pEVSL_DeviceInfo": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_size_t)]),
+    "pEVSL_SymmTridEigvals": (C.c_int, [C.c_int, c_double_p, c_double_p, C.c_double,
+                                        c_double_p, c_int_p, c_double_p]),
     "pEVSL_Autotune": (C.c_int, [DataP, C.c_int, C.POINTER(TuneResult)]),
     "pEVSL_BenchReorth": (C.c_int, [DataP, C.c_int, C.c_int, c_double_p, c_double_p]),
 }

@@ -222,6 +222,12 @@ int pEVSL_SetFilterPrecision(pevsl_Data *p, int prec) {
   return 0;
 }
 
+int pEVSL_SetRitzRefine(pevsl_Data *p, int mode) {
+  if (mode < -1 || mode > 1) return -1;
+  ctx_of(p).ritz_refine = mode;
+  return 0;
+}
+
 int pEVSL_SetMixedTol(pevsl_Data *p, double tol) {
   if (!(tol > 0)) return -1;
   ctx_of(p).mixed_tol = tol;
@@ -244,9 +250,15 @@ int pEVSL_SetKeepVectors(pevsl_Data *p, int keep) {
 
 int pEVSL_SetSpmvTuning(pevsl_Data *p, int lanes, int bw) {
   auto pow2 = [](int v) { return v >= 0 && v <= 64 && (v & (v - 1)) == 0; };
-  if (!pow2(lanes) || !pow2(bw)) return -1;
+  if (!(pow2(lanes) || lanes == PEVSL_SPMV_STREAM) || !pow2(bw)) return -1;
   Ctx &c = ctx_of(p);
-  c.lanes = lanes;
+  if (lanes == PEVSL_SPMV_STREAM) {
+    c.use_stream = 1;
+    c.lanes = 0;
+  } else {
+    c.use_stream = lanes ? 0 : -1;
+    c.lanes = lanes;
+  }
   c.block_width = bw;
   return 0;
 }
@@ -338,6 +350,11 @@ int pEVSL_ParcsrSetup(pevsl_Csr *Ai, pevsl_Parcsr *A) {
   int mr = 0;
   for (int i = 0; i < n; i++) mr = std::max(mr, Ai->ia[i + 1] - Ai->ia[i]);
   M->csr.max_row = mr;
+  std::vector<int> rb;
+  build_row_blocks(n, Ai->ia, rb);
+  M->csr.nrowblk = (int)rb.size() - 1;
+  M->csr.rowblk = M->rowblk.ensure(rb.size());
+  PV_HIP(hipMemcpy(M->csr.rowblk, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice));
   std::vector<float> a32((size_t)nnz);
   for (int j = 0; j < nnz; j++) a32[j] = (float)Ai->a[j];
   PV_HIP(hipMemcpy(M->csr.ia, Ai->ia, sizeof(int) * (size_t)(n + 1),

@@ -263,6 +263,29 @@ static void simpson_inplace(const double *xi, double *yi, int npts) {
   if (i == npts - 1) yi[i] = tc + (xi[i] - xi[i - 1]) * (yi[i] + ysav) / 2.0;
 }
 
+int pEVSL_SymmTridEigvals(int n, const double *diag, const double *sdiag, double thr,
+                          double *eigval, int *count, double *trace) {
+  if (n <= 0) return -1;
+  const hipStream_t s = aux().s;
+  DBuf<double> d;
+  double *dd = d.ensure((size_t)3 * n + 16);
+  double *de = dd + n, *lam = de + n, *out = lam + n;
+  PV_HIP(hipMemcpyAsync(dd, diag, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  if (n > 1)
+    PV_HIP(hipMemcpyAsync(de, sdiag, sizeof(double) * (n - 1), hipMemcpyHostToDevice, s));
+  tridiag_trace_above(n, dd, de, thr, lam, out, s);
+  double o[6];
+  PV_HIP(hipMemcpyAsync(o, out, sizeof(double) * 6, hipMemcpyDeviceToHost, s));
+  PV_HIP(hipStreamSynchronize(s));
+  const int first = (int)o[5];
+  *count = n - first;
+  *trace = o[0];
+  PV_HIP(hipMemcpyAsync(eigval, lam + first, sizeof(double) * (n - first),
+                        hipMemcpyDeviceToHost, s));
+  PV_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 void pEVSL_SpslicerLan(double *xi, double *yi, int n_int, int npts, double *sli) {
   simpson_inplace(xi, yi, npts);
   double t = yi[0];

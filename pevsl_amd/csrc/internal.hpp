@@ -61,7 +61,7 @@ struct DBuf {
 // Device copy of a CSR matrix (attached to pevsl_Parcsr.dev).
 struct DevMatrix {
   DevCsr csr;
-  DBuf<int> ia, ja;
+  DBuf<int> ia, ja, rowblk;
   DBuf<double> a64;
   DBuf<float> a32;
 };
@@ -76,10 +76,12 @@ struct Ctx {
   void *host_mv_data = nullptr;
   int filter_prec = PEVSL_PREC_FP64;
   double mixed_tol = 1e-5;         // residual threshold of the FP32-filter path
+  int ritz_refine = -1;            // FP64 Rayleigh-Ritz refinement: 1 on, 0 off, -1 FP32 only
   int rand_mode = PEVSL_RAND_LIBC;
   unsigned long long rand_seed = 0;
   unsigned long long rand_offset = 0;
   int lanes = 0;                   // lanes per row of the single-column SpMV
+  int use_stream = -1;             // CSR-stream filter kernel: 1 yes, 0 no, -1 auto
   int block_width = 0;             // columns per block SpMM (0 = auto)
   int keep_vectors = 1;
   int chunk_cols = 512;            // Lanczos basis allocation granularity

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace pevsl {
 
@@ -17,7 +18,18 @@ struct DevCsr {
   double *a64 = nullptr;
   float *a32 = nullptr;
   int max_row = 0;  // longest row
+  // row blocks of the CSR-stream kernels: consecutive rows with at most
+  // kStreamRows rows and kStreamNnz nonzeros (a longer row is a block alone)
+  int *rowblk = nullptr;
+  int nrowblk = 0;
 };
+
+constexpr int kStreamRows = 256;
+constexpr int kStreamNnz = 2048;
+
+// host: row-block partition for the CSR-stream kernels (returns nrowblk+1
+// offsets)
+void build_row_blocks(int n, const int *ia, std::vector<int> &blk);
 
 // ---- sparse products -------------------------------------------------------
 // y = A x, one column.  lanes = lanes per row in {1,2,4,8,16,32,64}.
@@ -34,6 +46,13 @@ template <typename T>
 void cheb_step_vec(const DevCsr &A, const T *vk, const T *vkm1, T *vkp1, T *y,
                    double cc, double t, double mu0, double muk, bool first,
                    int lanes, hipStream_t s);
+
+// Same step with the CSR-stream kernel (one block per row block: coalesced
+// products staged in LDS, then one sequential sum per row).
+template <typename T>
+void cheb_step_stream(const DevCsr &A, const T *vk, const T *vkm1, T *vkp1,
+                      T *y, double cc, double t, double mu0, double muk,
+                      bool first, hipStream_t s);
 
 // Same for B interleaved columns (row-major n x B block, column fastest).
 // mus: device array [(deg_max+1) x B] of per-column coefficients; k = degree

@@ -16,6 +16,7 @@ double step_bytes(const DevCsr &A, int elem, int ncol) {
   return (double)A.nnz * (elem + 4) + 4.0 * (A.n + 1) + 5.0 * (double)A.n * elem * ncol;
 }
 
+// lanes == 0: the CSR-stream kernel
 template <typename T>
 double time_vec(Ctx &c, int lanes, int reps) {
   const DevCsr &A = *c.A;
@@ -25,12 +26,18 @@ double time_vec(Ctx &c, int lanes, int reps) {
   T *v0 = b.p, *v1 = b.p + n, *v2 = b.p + 2 * (size_t)n, *y = b.p + 3 * (size_t)n;
   PV_HIP(hipMemsetAsync(b.p, 0, sizeof(T) * 4 * (size_t)n, c.stream));
   // warm-up
-  cheb_step_vec<T>(A, v0, v1, v2, y, 6.0, 1.0 / 6, 0.1, 0.1, false, lanes, c.stream);
+  auto step = [&](const T *vk, const T *vkm1, T *vkp1) {
+    if (lanes == 0)
+      cheb_step_stream<T>(A, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, 0.1, 0.1, false, c.stream);
+    else
+      cheb_step_vec<T>(A, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, 0.1, 0.1, false, lanes, c.stream);
+  };
+  step(v0, v1, v2);
   EvTimer ev;
   ev.start(c.stream);
   T *vk = v0, *vkm1 = v1, *vkp1 = v2;
   for (int r = 0; r < reps; r++) {
-    cheb_step_vec<T>(A, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, 0.1, 0.1, false, lanes, c.stream);
+    step(vk, vkm1, vkp1);
     T *t = vkm1;
     vkm1 = vk;
     vk = vkp1;
@@ -86,6 +93,7 @@ int pEVSL_Autotune(pevsl_Data *p, int reps, pevsl_TuneResult *res) {
     }
   }
   res->bytes_lanes = step_bytes(*c.A, elem, 1);
+  res->t_stream = f32 ? time_vec<float>(c, 0, reps) : time_vec<double>(c, 0, reps);
   DBuf<double> mus;
   mus.ensure(3 * 64);
   std::vector<double> mh(3 * 64, 0.01);
@@ -105,8 +113,9 @@ int pEVSL_Autotune(pevsl_Data *p, int reps, pevsl_TuneResult *res) {
       bw = B;
     }
   }
-  res->lanes = bl;
+  res->lanes = res->t_stream < best ? PEVSL_SPMV_STREAM : bl;
   res->block_width = bw;
+  c.use_stream = res->t_stream < best ? 1 : 0;
   c.lanes = bl;
   c.block_width = bw;
   return 0;

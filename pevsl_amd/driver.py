@@ -105,7 +105,7 @@ class Instance:
 
     def __init__(self, lib=None, matrix=None, prec=capi.PEVSL_PREC_FP64,
                  rand_mode=capi.PEVSL_RAND_LIBC, seed=0, keep_vectors=False,
-                 lanes=0, block_width=0):
+                 lanes=0, block_width=0, ritz_refine=-1, mixed_tol=None):
         self.lib = lib or capi.load()
         self.p = capi.DataP()
         rc = self.lib.pEVSL_Start(capi.MPI_COMM_WORLD, C.byref(self.p))
@@ -117,6 +117,9 @@ class Instance:
         self.lib.pEVSL_SetRandMode(self.p, rand_mode, seed)
         self.lib.pEVSL_SetKeepVectors(self.p, 1 if keep_vectors else 0)
         self.lib.pEVSL_SetSpmvTuning(self.p, lanes, block_width)
+        self.lib.pEVSL_SetRitzRefine(self.p, ritz_refine)
+        if mixed_tol is not None:
+            self.lib.pEVSL_SetMixedTol(self.p, mixed_tol)
 
     def close(self):
         if self.p:
@@ -262,7 +265,7 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
                     prec=capi.PEVSL_PREC_FP64, batch=False, slices_only=None,
                     keep_vectors=False, rand_mode=capi.PEVSL_RAND_LIBC, seed=0,
                     sli_override=None, block_width=0, log=None, verbose=False,
-                    maxit_cap=None):
+                    maxit_cap=None, lanes=0, ritz_refine=-1, mixed_tol=None):
     """End-to-end run of the TESTS/Lap/LapPLanN.c flow on the GPU.  Returns a
     dict with the DOS, slices, per-slice filters, eigenvalues and timings."""
     lib = capi.load()
@@ -273,7 +276,8 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
     lmin, lmax = 0.0, (8.0 if nz == 1 else 12.0)
     tol = 1e-8
     inst = Instance(lib, M, prec=prec, rand_mode=rand_mode, seed=seed,
-                    block_width=block_width)
+                    block_width=block_width, lanes=lanes, ritz_refine=ritz_refine,
+                    mixed_tol=mixed_tol)
     out = {"n": n, "nnz": M.nnz, "t_setup": time.time() - t0}
     try:
         xintv = np.array([a, b, lmin, lmax])

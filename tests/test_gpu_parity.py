@@ -69,7 +69,9 @@ def test_device_matvec_sin_recipe(lib, lap20, lanes):
         assert np.max(np.abs(y - ref)) <= 1e-14 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("lanes,prec", [(1, 0), (2, 0), (8, 0), (2, 1)])
+@pytest.mark.parametrize("lanes,prec", [(1, 0), (2, 0), (8, 0), (2, 1),
+                                        (capi.PEVSL_SPMV_STREAM, 0),
+                                        (capi.PEVSL_SPMV_STREAM, 1)])
 def test_cheb_av_against_oracle(lib, lap20, lanes, prec):
     banner("ChebAv 20^3 lanes=%d prec=%d" % (lanes, prec))
     M, A = lap20
@@ -92,7 +94,8 @@ def test_cheb_av_against_oracle(lib, lap20, lanes, prec):
     if prec == 1:
         # FP32 filter: error relative to the filtered vector's size
         assert np.max(np.abs(y - ref)) <= 1e-4 * scale
-    elif lanes == 1:
+    elif lanes in (1, capi.PEVSL_SPMV_STREAM):
+        # one sequential sum per row: same rounding as the reference loop
         assert np.array_equal(y, ref)
     else:
         assert np.max(np.abs(y - ref)) <= 1e-12 * scale
@@ -282,3 +285,67 @@ def test_reorth_kernel_orthogonality(lib):
     Y = s["vecs"]
     G = Y @ Y.T
     assert np.max(np.abs(G - np.eye(len(Y)))) < 1e-8
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 300, 2000])
+def test_tridiag_multisection_against_lapack(lib, n):
+    banner("GPU Sturm multisection n=%d" % n)
+    import scipy.linalg as sla
+    rng = np.random.default_rng(n)
+    d = rng.uniform(-1, 1, n)
+    e = rng.uniform(0.0, 1, max(n - 1, 0))
+    if n > 10:
+        e[5] = 0.0          # split matrix
+        d[20:40] = 0.3      # a cluster
+        e[20:39] = 1e-9
+    ref = sla.eigh_tridiagonal(d, e, eigvals_only=True)
+    lam = np.zeros(n)
+    cnt = C.c_int()
+    tr = C.c_double()
+    rc = lib.pEVSL_SymmTridEigvals(n, d.ctypes.data_as(capi.c_double_p),
+                                   e.ctypes.data_as(capi.c_double_p), -np.inf,
+                                   lam.ctypes.data_as(capi.c_double_p), C.byref(cnt),
+                                   C.byref(tr))
+    assert rc == 0 and cnt.value == n
+    # bisection and LAPACK's MRRR each have O(sqrt(n) eps |T|) absolute error
+    scale = max(1.0, np.max(np.abs(ref)))
+    assert np.max(np.abs(lam - ref)) <= 64 * np.finfo(float).eps * scale * np.sqrt(n)
+    # restricted trace above a threshold (the ChebLanNr stopping test)
+    thr = 0.25
+    rc = lib.pEVSL_SymmTridEigvals(n, d.ctypes.data_as(capi.c_double_p),
+                                   e.ctypes.data_as(capi.c_double_p), thr,
+                                   lam.ctypes.data_as(capi.c_double_p), C.byref(cnt),
+                                   C.byref(tr))
+    sel = ref[ref >= thr]
+    assert cnt.value == len(sel)
+    assert abs(tr.value - sel.sum()) <= 64 * np.finfo(float).eps * scale * np.sqrt(n) * max(1, len(sel))
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_mixed_precision_c1(lib, batch):
+    banner("C1 mixed precision (FP32 filter + FP64 Rayleigh-Ritz) batch=%s" % batch)
+    # stated mixed-path tolerance: residual < 1e-5, eigenvalue error < 1e-10
+    r = driver.lap_slicing_run(20, 20, 20, 0.6, 1.2, 5, batch=batch, prec=capi.PEVSL_PREC_FP32)
+    for s in r["slices"]:
+        assert s["nev"] == s["nev_exact"]
+        assert s["max_rel_err"] < 1e-10
+        assert np.all(s["res"] < 1e-5)
+
+
+def test_fp64_rayleigh_ritz_refinement(lib):
+    banner("C1 FP64 with the Rayleigh-Ritz Ritz stage")
+    r = driver.lap_slicing_run(20, 20, 20, 0.6, 1.2, 5, batch=True, ritz_refine=1)
+    for s in r["slices"]:
+        assert s["nev"] == s["nev_exact"]
+        assert s["max_rel_err"] < 1e-12
+        assert np.all(s["res"] < 1e-8)
+
+
+def test_stream_kernel_end_to_end(lib):
+    banner("C1 FP64 with the CSR-stream filter kernel")
+    r = driver.lap_slicing_run(20, 20, 20, 0.6, 1.2, 5, lanes=capi.PEVSL_SPMV_STREAM)
+    d = orc.run_driver(["-nx", 20, "-ny", 20, "-nz", 20, "-a", 0.6, "-b", 1.2,
+                        "-nslices", 5], timeout=120)
+    for s, ds in zip(r["slices"], d["slices"]):
+        assert s["nev"] == s["nev_exact"] == ds["nev"]
+        assert s["info"]["its"] == ds["its"]
