@@ -88,6 +88,10 @@ int pEVSL_SetKeepVectors(pevsl_Data *pevsl, int keep);
 #define PEVSL_SPMV_STREAM 128
 int pEVSL_SetSpmvTuning(pevsl_Data *pevsl, int lanes_per_row, int block_width);
 
+/* Block SpMM kernel of batched solves: 1 = one thread per row with B-wide
+ * row gathers (default, B <= 32), 0 = one lane per (row, column). */
+int pEVSL_SetBlockKernel(pevsl_Data *pevsl, int rowkernel);
+
 /* Eigenvalues >= thr of the symmetric tridiagonal matrix (diag[0..n-1],
  * sdiag[0..n-2]) by Sturm multisection on the GPU (thr = -HUGE_VAL: all), in
  * ascending order in eigval[0..*count-1], and their sum in *trace.  The
@@ -143,6 +147,9 @@ typedef struct {
   double t_block[6];
   double bytes_block[6];
   double t_stream;    /* CSR-stream kernel, seconds per step */
+  double t_blockrow[6]; /* row-per-thread block kernel, widths 2..64 (0: n/a) */
+  int block_rowkernel;  /* chosen block kernel: 1 row-per-thread, 0 lane per
+                           (row, column) */
 } pevsl_TuneResult;
 int pEVSL_Autotune(pevsl_Data *pevsl, int reps, pevsl_TuneResult *res);
 

@@ -86,7 +86,8 @@ class TuneResult(C.Structure):
     _fields_ = [("lanes", C.c_int), ("block_width", C.c_int),
                 ("t_lanes", C.c_double * 6), ("bytes_lanes", C.c_double),
                 ("t_block", C.c_double * 6), ("bytes_block", C.c_double * 6),
-                ("t_stream", C.c_double)]
+                ("t_stream", C.c_double), ("t_blockrow", C.c_double * 6),
+                ("block_rowkernel", C.c_int)]
 
 
 class KernelStats(C.Structure):
@@ -214,6 +215,7 @@ SIGNATURES = {
     "pEVSL_SetRitzRefine": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetRandMode": (C.c_int, [DataP, C.c_int, C.c_ulonglong]),
     "pEVSL_SetKeepVectors": (C.c_int, [DataP, C.c_int]),
+    "pEVSL_SetBlockKernel": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetSpmvTuning": (C.c_int, [DataP, C.c_int, C.c_int]),
     "pEVSL_ChebLanNrBatch": (C.c_int, [DataP, C.c_int, c_double_p, c_int_p,
                                        C.c_double, C.POINTER(Parvec),

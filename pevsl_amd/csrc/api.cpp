@@ -263,6 +263,12 @@ int pEVSL_SetSpmvTuning(pevsl_Data *p, int lanes, int bw) {
   return 0;
 }
 
+int pEVSL_SetBlockKernel(pevsl_Data *p, int rowkernel) {
+  if (rowkernel < -1 || rowkernel > 1) return -1;
+  ctx_of(p).blk_rowkernel = rowkernel;
+  return 0;
+}
+
 int pEVSL_GetLanInfo(pevsl_Data *p, int s, pevsl_LanInfo *info) {
   Ctx &c = ctx_of(p);
   if (s < 0 || s >= (int)c.info.size()) return -1;

@@ -83,6 +83,7 @@ struct Ctx {
   int lanes = 0;                   // lanes per row of the single-column SpMV
   int use_stream = -1;             // CSR-stream filter kernel: 1 yes, 0 no, -1 auto
   int block_width = 0;             // columns per block SpMM (0 = auto)
+  int blk_rowkernel = -1;          // block SpMM: 1 row-per-thread, 0 lane per (row, col), -1 auto
   int keep_vectors = 1;
   int chunk_cols = 512;            // Lanczos basis allocation granularity
   std::vector<pevsl_LanInfo> info; // per slice of the last (batched) solve

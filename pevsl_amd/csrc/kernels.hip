@@ -217,6 +217,95 @@ k_cheb_blk(int n, const int *__restrict__ ia, const int *__restrict__ ja,
   }
 }
 
+// Row-per-thread block step for B interleaved columns: the row block's
+// a/ja are staged in LDS (coalesced), then each thread owns one row and
+// gathers whole B-wide rows of vk with 16-byte loads, so one memory
+// instruction moves B values instead of one.  Columns >= nact keep y.
+template <typename T, int B>
+__device__ __forceinline__ void load_row(const T *__restrict__ p, T (&x)[B]) {
+  if constexpr ((B * sizeof(T)) % 16 == 0) {
+    const float4 *q = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+    for (int i = 0; i < (int)(B * sizeof(T) / 16); i++)
+      reinterpret_cast<float4 *>(x)[i] = q[i];
+  } else {
+#pragma unroll
+    for (int c = 0; c < B; c++) x[c] = p[c];
+  }
+}
+template <typename T, int B>
+__device__ __forceinline__ void store_row(T *__restrict__ p, const T (&x)[B]) {
+  if constexpr ((B * sizeof(T)) % 16 == 0) {
+    float4 *q = reinterpret_cast<float4 *>(p);
+#pragma unroll
+    for (int i = 0; i < (int)(B * sizeof(T) / 16); i++)
+      q[i] = reinterpret_cast<const float4 *>(x)[i];
+  } else {
+#pragma unroll
+    for (int c = 0; c < B; c++) p[c] = x[c];
+  }
+}
+
+template <typename T, int B>
+__global__ void __launch_bounds__(kThreads)
+k_cheb_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
+              const int *__restrict__ ja, const T *__restrict__ a,
+              const T *__restrict__ vk, const T *__restrict__ vkm1,
+              T *__restrict__ vkp1, T *__restrict__ y, T cc, T t,
+              const double *__restrict__ mus, int k, int nact) {
+#pragma clang fp contract(off)
+  __shared__ T as[kStreamNnz];
+  __shared__ int js[kStreamNnz];
+  const int r0 = rowblk[blockIdx.x], r1 = rowblk[blockIdx.x + 1];
+  const int j0 = ia[r0], nz = ia[r1] - j0;
+  const bool staged = nz <= kStreamNnz;
+  if (staged) {
+    for (int q = threadIdx.x; q < nz; q += kThreads) {
+      as[q] = a[j0 + q];
+      js[q] = ja[j0 + q];
+    }
+    __syncthreads();
+  } else if (threadIdx.x != 0) {
+    return;  // one long row: handled by thread 0 from global memory
+  }
+  const int r = r0 + threadIdx.x;
+  if (r >= r1) return;
+  T acc[B];
+#pragma unroll
+  for (int c = 0; c < B; c++) acc[c] = 0;
+  const int qs = ia[r] - j0, qe = ia[r + 1] - j0;
+  for (int q = qs; q < qe; q++) {
+    const T av = staged ? as[q] : a[j0 + q];
+    const int col = staged ? js[q] : ja[j0 + q];
+    T x[B];
+    load_row<T, B>(vk + (size_t)col * B, x);
+#pragma unroll
+    for (int c = 0; c < B; c++) acc[c] += av * x[c];
+  }
+  T v[B], vm[B], yy[B], w[B];
+  load_row<T, B>(vk + (size_t)r * B, v);
+  load_row<T, B>(y + (size_t)r * B, yy);
+  if (k == 1) {
+#pragma unroll
+    for (int c = 0; c < B; c++) {
+      w[c] = acc[c] + (-cc) * v[c];
+      w[c] = w[c] * t;
+      if (c < nact) yy[c] = v[c] * (T)mus[c] + (T)mus[(size_t)B + c] * w[c];
+    }
+  } else {
+    load_row<T, B>(vkm1 + (size_t)r * B, vm);
+#pragma unroll
+    for (int c = 0; c < B; c++) {
+      w[c] = acc[c] + (-cc) * v[c];
+      w[c] = w[c] * t;
+      w[c] = w[c] + (T)(-1) * vm[c];
+      if (c < nact) yy[c] = yy[c] + (T)mus[(size_t)k * B + c] * w[c];
+    }
+  }
+  store_row<T, B>(vkp1 + (size_t)r * B, w);
+  store_row<T, B>(y + (size_t)r * B, yy);
+}
+
 // KPM step + per-column partial dot with v0 over the block's rows.
 template <typename T, int B>
 __global__ void __launch_bounds__(kThreads)
@@ -942,6 +1031,33 @@ void cheb_step_blk(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
       break;
   }
 }
+template <typename T>
+void cheb_step_blkrow(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
+                      T *y, double cc, double t, const double *mus, int k,
+                      int nact, hipStream_t s) {
+  if (A.n == 0 || nact <= 0 || A.nrowblk == 0) return;
+  const T *a = std::is_same<T, double>::value ? (const T *)A.a64
+                                              : (const T *)A.a32;
+  switch (B) {
+#define CASE(BB)                                                                  \
+  case BB:                                                                        \
+    hipLaunchKernelGGL((k_cheb_blkrow<T, BB>), dim3(A.nrowblk), dim3(kThreads), 0, \
+                       s, A.rowblk, A.ia, A.ja, a, vk, vkm1, vkp1, y, (T)cc, (T)t,  \
+                       mus, k, nact);                                             \
+    break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32)
+#undef CASE
+    default:
+      break;
+  }
+}
+template void cheb_step_blkrow<double>(const DevCsr &, int, const double *,
+                                       const double *, double *, double *, double,
+                                       double, const double *, int, int, hipStream_t);
+template void cheb_step_blkrow<float>(const DevCsr &, int, const float *, const float *,
+                                      float *, float *, double, double, const double *,
+                                      int, int, hipStream_t);
+
 template void cheb_step_blk<double>(const DevCsr &, int, const double *,
                                     const double *, double *, double *, double,
                                     double, const double *, int, int,

@@ -62,6 +62,13 @@ void cheb_step_blk(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
                    T *y, double cc, double t, const double *mus, int k,
                    int nact, hipStream_t s);
 
+// Same with one thread per row (row blocks of the CSR-stream partition,
+// a/ja staged in LDS, B-wide 16-byte row gathers); B <= 32.
+template <typename T>
+void cheb_step_blkrow(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
+                      T *y, double cc, double t, const double *mus, int k,
+                      int nact, hipStream_t s);
+
 // KPM step on B interleaved columns:
 //   vkp1 = scal (A vk - ctr vk) - vkm1,   part[blk][c] = sum_rows vkp1*v0
 // returns the number of row blocks (partials per column) written.

@@ -296,6 +296,8 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
   const double cc = S_[(size_t)grp[0]].pol->cc, dd = S_[(size_t)grp[0]].pol->dd;
   std::vector<int> degs;
   for (int s : grp) degs.push_back(S_[(size_t)s].pol->deg);
+  // row-per-thread kernel (default) up to 32 columns
+  const bool rowk = c_.blk_rowkernel != 0 && B <= 32 && A.nrowblk > 0;
   auto run = [&](auto *tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
     T *x = (T *)xb[0], *y = (T *)xb[1];
@@ -315,8 +317,12 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
         vkp1 = bufs[nb];
       }
       nb = (nb + 1) % 3;
-      cheb_step_blk<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd,
-                       mus_.p, d, nact, c_.stream);
+      if (rowk)
+        cheb_step_blkrow<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd,
+                            mus_.p, d, nact, c_.stream);
+      else
+        cheb_step_blk<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd,
+                         mus_.p, d, nact, c_.stream);
       c_.kstats.filter_bytes += (double)filter_launch_bytes(A, (int)elem, nact, d == 1);
       c_.kstats.filter_steps += nact;
       c_.kstats.filter_launches++;

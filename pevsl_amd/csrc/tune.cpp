@@ -48,19 +48,25 @@ double time_vec(Ctx &c, int lanes, int reps) {
 }
 
 template <typename T>
-double time_blk(Ctx &c, int B, int reps, DBuf<double> &mus) {
+double time_blk(Ctx &c, int B, int reps, DBuf<double> &mus, bool rowk) {
   const DevCsr &A = *c.A;
   const size_t blk = (size_t)A.n * B;
   DBuf<T> b;
   b.ensure(4 * blk);
   PV_HIP(hipMemsetAsync(b.p, 0, sizeof(T) * 4 * blk, c.stream));
   T *v0 = b.p, *v1 = b.p + blk, *v2 = b.p + 2 * blk, *y = b.p + 3 * blk;
-  cheb_step_blk<T>(A, B, v0, v1, v2, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
+  auto step = [&](const T *vk, const T *vkm1, T *vkp1) {
+    if (rowk)
+      cheb_step_blkrow<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
+    else
+      cheb_step_blk<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
+  };
+  step(v0, v1, v2);
   EvTimer ev;
   ev.start(c.stream);
   T *vk = v0, *vkm1 = v1, *vkp1 = v2;
   for (int r = 0; r < reps; r++) {
-    cheb_step_blk<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
+    step(vk, vkm1, vkp1);
     T *t = vkm1;
     vkm1 = vk;
     vk = vkp1;
@@ -99,18 +105,30 @@ int pEVSL_Autotune(pevsl_Data *p, int reps, pevsl_TuneResult *res) {
   std::vector<double> mh(3 * 64, 0.01);
   PV_HIP(hipMemcpy(mus.p, mh.data(), sizeof(double) * mh.size(), hipMemcpyHostToDevice));
   double bestc = 1e30;
-  int bw = 1;
+  int bw = 1, rowk = 1;
   for (int i = 0; i < 6; i++) {
     const int B = 2 << i;
     // skip widths whose blocks would not fit comfortably (4 blocks)
     const double need = 4.0 * c.A->n * (double)B * elem;
     if (need > 32e9) break;
-    const double t = f32 ? time_blk<float>(c, B, reps, mus) : time_blk<double>(c, B, reps, mus);
+    const double t = f32 ? time_blk<float>(c, B, reps, mus, false)
+                         : time_blk<double>(c, B, reps, mus, false);
     res->t_block[i] = t;
     res->bytes_block[i] = step_bytes(*c.A, elem, B);
     if (t / B < bestc) {
       bestc = t / B;
       bw = B;
+      rowk = 0;
+    }
+    if (B <= 32) {
+      const double tr = f32 ? time_blk<float>(c, B, reps, mus, true)
+                            : time_blk<double>(c, B, reps, mus, true);
+      res->t_blockrow[i] = tr;
+      if (tr / B < bestc) {
+        bestc = tr / B;
+        bw = B;
+        rowk = 1;
+      }
     }
   }
   res->lanes = res->t_stream < best ? PEVSL_SPMV_STREAM : bl;
@@ -118,6 +136,8 @@ int pEVSL_Autotune(pevsl_Data *p, int reps, pevsl_TuneResult *res) {
   c.use_stream = res->t_stream < best ? 1 : 0;
   c.lanes = bl;
   c.block_width = bw;
+  c.blk_rowkernel = rowk;
+  res->block_rowkernel = rowk;
   return 0;
 }
 

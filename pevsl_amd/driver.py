@@ -105,7 +105,8 @@ class Instance:
 
     def __init__(self, lib=None, matrix=None, prec=capi.PEVSL_PREC_FP64,
                  rand_mode=capi.PEVSL_RAND_LIBC, seed=0, keep_vectors=False,
-                 lanes=0, block_width=0, ritz_refine=-1, mixed_tol=None):
+                 lanes=0, block_width=0, ritz_refine=-1, mixed_tol=None,
+                 block_rowkernel=-1):
         self.lib = lib or capi.load()
         self.p = capi.DataP()
         rc = self.lib.pEVSL_Start(capi.MPI_COMM_WORLD, C.byref(self.p))
@@ -118,6 +119,7 @@ class Instance:
         self.lib.pEVSL_SetKeepVectors(self.p, 1 if keep_vectors else 0)
         self.lib.pEVSL_SetSpmvTuning(self.p, lanes, block_width)
         self.lib.pEVSL_SetRitzRefine(self.p, ritz_refine)
+        self.lib.pEVSL_SetBlockKernel(self.p, block_rowkernel)
         if mixed_tol is not None:
             self.lib.pEVSL_SetMixedTol(self.p, mixed_tol)
 
@@ -265,7 +267,8 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
                     prec=capi.PEVSL_PREC_FP64, batch=False, slices_only=None,
                     keep_vectors=False, rand_mode=capi.PEVSL_RAND_LIBC, seed=0,
                     sli_override=None, block_width=0, log=None, verbose=False,
-                    maxit_cap=None, lanes=0, ritz_refine=-1, mixed_tol=None):
+                    maxit_cap=None, lanes=0, ritz_refine=-1, mixed_tol=None,
+                    block_rowkernel=-1):
     """End-to-end run of the TESTS/Lap/LapPLanN.c flow on the GPU.  Returns a
     dict with the DOS, slices, per-slice filters, eigenvalues and timings."""
     lib = capi.load()
@@ -277,7 +280,7 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
     tol = 1e-8
     inst = Instance(lib, M, prec=prec, rand_mode=rand_mode, seed=seed,
                     block_width=block_width, lanes=lanes, ritz_refine=ritz_refine,
-                    mixed_tol=mixed_tol)
+                    mixed_tol=mixed_tol, block_rowkernel=block_rowkernel)
     out = {"n": n, "nnz": M.nnz, "t_setup": time.time() - t0}
     try:
         xintv = np.array([a, b, lmin, lmax])

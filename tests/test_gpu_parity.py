@@ -200,13 +200,16 @@ def test_c1_readme_slices(lib):
         assert s["nev"] == gs["found"] == gs["exact"]
 
 
-def test_batch_equals_single_slices(lib):
-    banner("batched vs one-by-one solve, 16^3, 4 slices")
+@pytest.mark.parametrize("rowk", [0, 1])
+def test_batch_equals_single_slices(lib, rowk):
+    banner("batched vs one-by-one solve, 16^3, 4 slices, block kernel rowk=%d" % rowk)
     r1 = driver.lap_slicing_run(16, 16, 16, 0.4, 1.0, 4, batch=False)
-    r2 = driver.lap_slicing_run(16, 16, 16, 0.4, 1.0, 4, batch=True)
+    r2 = driver.lap_slicing_run(16, 16, 16, 0.4, 1.0, 4, batch=True, block_rowkernel=rowk)
     for a, b in zip(r1["slices"], r2["slices"]):
         assert a["nev"] == b["nev"] == a["nev_exact"]
         np.testing.assert_allclose(a["lam"], b["lam"], rtol=1e-12)
+        # block kernels sum each row sequentially: same Lanczos run
+        assert a["info"]["its"] == b["info"]["its"]
 
 
 def test_lanczos_dos_against_oracle(lib):
