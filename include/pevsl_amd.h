@@ -159,6 +159,80 @@ int pEVSL_Autotune(pevsl_Data *pevsl, int reps, pevsl_TuneResult *res);
 int pEVSL_BenchReorth(pevsl_Data *pevsl, int k, int reps, double *sec,
                       double *bytes);
 
+/* ---- Learned slicer (no counterpart in the reference) -------------------
+ * A fully connected network (GELU hidden layers) trained and evaluated by
+ * this library's own HIP kernels proposes the density of states of an
+ * interval from a cheap KPM run; slices and the eigenvalue count follow from
+ * it, filter degrees from pEVSL_FindPol on the proposed slices.  Full KPM
+ * (pEVSL_Kpmdos) produces the training targets and judges a proposal
+ * (pEVSL_SlicerJudge); pEVSL_SpslicerKpm is the fallback.  It stands in for
+ * the outputs of pEVSL_Kpmdos (INC/pevsl.h:17-18) and pEVSL_SpslicerKpm
+ * (INC/pevsl.h:133). */
+typedef struct _pevsl_Mlp pevsl_Mlp;
+
+/* nlayers hidden layers of width nhidden between nin inputs and nout
+ * outputs.  The output z is added to a per-sample prior q (NULL: zero); the
+ * first nsoft values of y = z + q are a softmax group (cross-entropy
+ * against target masses), the rest are regressions (squared error).  Hidden
+ * weights are He-normal from `seed`, the output layer starts at zero. */
+int pEVSL_MlpCreate(int nin, int nhidden, int nlayers, int nout, int nsoft,
+                    unsigned long long seed, pevsl_Mlp **mlp);
+int pEVSL_MlpFree(pevsl_Mlp *mlp);
+int pEVSL_MlpShape(pevsl_Mlp *mlp, int *nin, int *nhidden, int *nlayers, int *nout,
+                   int *nsoft, size_t *nweights);
+/* Parameters packed layer by layer: W_l (out x in, row-major), then b_l. */
+int pEVSL_MlpSetWeights(pevsl_Mlp *mlp, const float *w, size_t nw);
+int pEVSL_MlpGetWeights(pevsl_Mlp *mlp, float *w, size_t nw);
+/* Y[batch x nout]: softmax probabilities, then regression values.
+ * X is batch x nin and Q batch x nout, row-major host arrays. */
+int pEVSL_MlpForward(pevsl_Mlp *mlp, const float *X, const float *Q, int batch, float *Y);
+/* Mean loss over the batch and its gradient (nweights floats, may be NULL). */
+int pEVSL_MlpLossGrad(pevsl_Mlp *mlp, const float *X, const float *Q, const float *T,
+                      int batch, float *loss, float *grad);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) at learning rate lr. */
+int pEVSL_MlpTrainStep(pevsl_Mlp *mlp, const float *X, const float *Q, const float *T,
+                       int batch, float lr, float *loss);
+/* `epochs` shuffled passes over a device-resident training set, minibatches
+ * of `batch`, Adam with cosine decay of lr to 2%; losses[e] = mean loss of
+ * epoch e (may be NULL). */
+int pEVSL_MlpTrain(pevsl_Mlp *mlp, const float *X, const float *Q, const float *T,
+                   int nsamples, int batch, int epochs, float lr, unsigned long long seed,
+                   float *losses);
+int pEVSL_MlpSave(pevsl_Mlp *mlp, const char *path);
+int pEVSL_MlpLoad(const char *path, pevsl_Mlp **mlp);
+
+#define PEVSL_SLICER_MDEG 20   /* cheap KPM degree */
+#define PEVSL_SLICER_NVEC 8    /* cheap KPM random vectors */
+#define PEVSL_SLICER_NBINS 512 /* DOS bins over the interval */
+#define PEVSL_SLICER_NFREQ 24  /* Fourier features per interval end */
+#define PEVSL_SLICER_NIN (42 + 4 * PEVSL_SLICER_NFREQ) /* network inputs */
+/* Network output layout: NBINS softmax logits, then log(fraction of the
+ * spectrum inside the interval). */
+
+/* Fractions of the spectrum in nbins equal bins of [intv[0], intv[1]]
+ * (clipped to the frame [intv[2], intv[3]]) from KPM moments mu[0..Mdeg] in
+ * the convention of pEVSL_Kpmdos; *frac = their sum. */
+int pEVSL_KpmBinMasses(const double *mu, int Mdeg, const double *intv, int nbins,
+                       double *masses, double *frac);
+/* Equal-share cuts of a binned DOS over [intv[0], intv[1]] (linear inside a
+ * bin): sli[0..n_int]. */
+int pEVSL_SlicerFromBins(const double *masses, int nbins, const double *intv, int n_int,
+                         double *sli);
+/* Network inputs (PEVSL_SLICER_NIN) and prior (NBINS + 1) from cheap KPM
+ * moments mu[0..Mdeg], Mdeg >= PEVSL_SLICER_MDEG (the first MDEG are used). */
+int pEVSL_SlicerFeaturesFromMu(const double *mu, int Mdeg, int n, const double *intv,
+                               float *feat, float *prior);
+/* The same after running the cheap KPM on the instance's matrix. */
+int pEVSL_SlicerFeatures(pevsl_Data *pevsl, const double *intv, float *feat, float *prior);
+/* Proposed slices sli[0..n_int] of intv[0..3] and the eigenvalue count in
+ * [intv[0], intv[1]]; returns -4 when mlp is not a slicer network. */
+int pEVSL_SpslicerMlp(pevsl_Data *pevsl, pevsl_Mlp *mlp, const double *intv, int n_int,
+                      double *sli, double *ecnt);
+/* Judge: counts[j] = n x (KPM fraction of the spectrum in slice j) from the
+ * moments of a full pEVSL_Kpmdos run, *imbalance = max count / mean - 1. */
+int pEVSL_SlicerJudge(const double *mu, int Mdeg, const double *intv, double n, int n_int,
+                      const double *sli, double *counts, double *imbalance);
+
 #ifdef __cplusplus
 }
 #endif

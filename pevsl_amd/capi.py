@@ -18,9 +18,15 @@ PEVSL_RAND_LIBC = 0
 PEVSL_RAND_RADEMACHER = 1
 PEVSL_RAND_UNIFORM = 2
 PEVSL_SPMV_STREAM = 128
+PEVSL_SLICER_MDEG = 20
+PEVSL_SLICER_NVEC = 8
+PEVSL_SLICER_NBINS = 512
+PEVSL_SLICER_NFREQ = 24
+PEVSL_SLICER_NIN = 42 + 4 * PEVSL_SLICER_NFREQ
 
 c_double_p = C.POINTER(C.c_double)
 c_int_p = C.POINTER(C.c_int)
+c_float_p = C.POINTER(C.c_float)
 
 
 class Csr(C.Structure):
@@ -233,6 +239,34 @@ SIGNATURES = {
                                         c_double_p, c_int_p, c_double_p]),
     "pEVSL_Autotune": (C.c_int, [DataP, C.c_int, C.POINTER(TuneResult)]),
     "pEVSL_BenchReorth": (C.c_int, [DataP, C.c_int, C.c_int, c_double_p, c_double_p]),
+    # learned slicer
+    "pEVSL_MlpCreate": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.c_ulonglong, C.POINTER(C.c_void_p)]),
+    "pEVSL_MlpFree": (C.c_int, [C.c_void_p]),
+    "pEVSL_MlpShape": (C.c_int, [C.c_void_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p,
+                                 C.POINTER(C.c_size_t)]),
+    "pEVSL_MlpSetWeights": (C.c_int, [C.c_void_p, c_float_p, C.c_size_t]),
+    "pEVSL_MlpGetWeights": (C.c_int, [C.c_void_p, c_float_p, C.c_size_t]),
+    "pEVSL_MlpForward": (C.c_int, [C.c_void_p, c_float_p, c_float_p, C.c_int, c_float_p]),
+    "pEVSL_MlpLossGrad": (C.c_int, [C.c_void_p, c_float_p, c_float_p, c_float_p, C.c_int,
+                                    c_float_p, c_float_p]),
+    "pEVSL_MlpTrainStep": (C.c_int, [C.c_void_p, c_float_p, c_float_p, c_float_p, C.c_int,
+                                     C.c_float, c_float_p]),
+    "pEVSL_MlpTrain": (C.c_int, [C.c_void_p, c_float_p, c_float_p, c_float_p, C.c_int,
+                                 C.c_int, C.c_int, C.c_float, C.c_ulonglong, c_float_p]),
+    "pEVSL_MlpSave": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "pEVSL_MlpLoad": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "pEVSL_KpmBinMasses": (C.c_int, [c_double_p, C.c_int, c_double_p, C.c_int, c_double_p,
+                                     c_double_p]),
+    "pEVSL_SlicerFromBins": (C.c_int, [c_double_p, C.c_int, c_double_p, C.c_int,
+                                       c_double_p]),
+    "pEVSL_SlicerFeaturesFromMu": (C.c_int, [c_double_p, C.c_int, C.c_int, c_double_p,
+                                             c_float_p, c_float_p]),
+    "pEVSL_SlicerFeatures": (C.c_int, [DataP, c_double_p, c_float_p, c_float_p]),
+    "pEVSL_SpslicerMlp": (C.c_int, [DataP, C.c_void_p, c_double_p, C.c_int, c_double_p,
+                                    c_double_p]),
+    "pEVSL_SlicerJudge": (C.c_int, [c_double_p, C.c_int, c_double_p, C.c_double, C.c_int,
+                                    c_double_p, c_double_p, c_double_p]),
 }
 
 _lib = None

@@ -268,9 +268,15 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
                     keep_vectors=False, rand_mode=capi.PEVSL_RAND_LIBC, seed=0,
                     sli_override=None, block_width=0, log=None, verbose=False,
                     maxit_cap=None, lanes=0, ritz_refine=-1, mixed_tol=None,
-                    block_rowkernel=-1):
+                    block_rowkernel=-1, mlp=None, judge=True, judge_tol=0.15):
     """End-to-end run of the TESTS/Lap/LapPLanN.c flow on the GPU.  Returns a
-    dict with the DOS, slices, per-slice filters, eigenvalues and timings."""
+    dict with the DOS, slices, per-slice filters, eigenvalues and timings.
+
+    dostype 0: KPM (Mdeg, nvec) + SpslicerKpm; 1: Lanczos DOS + SpslicerLan;
+    3: learned slicer (`mlp`: a slicer.Mlp or a weights path).  With
+    `judge`, dostype 3 also runs the full KPM, and when the full-KPM counts of
+    the proposed slices are more than `judge_tol` above their mean it falls
+    back to SpslicerKpm on the full moments."""
     lib = capi.load()
     t0 = time.time()
     ia, ja, av = laplacian3d(nx, ny, nz)
@@ -294,6 +300,30 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
             if rc:
                 raise RuntimeError("spslicer error %d" % rc)
             out["mu"] = mu
+        elif dostype == 3:
+            from . import slicer as slc
+            net = slc.Mlp.load(mlp, lib) if isinstance(mlp, str) else mlp
+            try:
+                sli, ecount = slc.spslicer_mlp(inst, net, xintv, nslices)
+            finally:
+                if isinstance(mlp, str):
+                    net.free()
+            out["t_slicer"] = time.time() - t1
+            if judge:
+                rc, mu, ecount_full = inst.kpmdos(Mdeg, 1, nvec, xintv)
+                if rc:
+                    raise RuntimeError("kpmdos error %d" % rc)
+                cnt, imb = slc.judge(lib, mu, xintv, n, sli)
+                fallback = imb > judge_tol
+                out["judge"] = {"counts": cnt.tolist(), "imbalance": imb,
+                                "ecount_full": ecount_full, "ecount_mlp": ecount,
+                                "fallback": bool(fallback)}
+                if fallback:
+                    rc, sli = spslicer_kpm(lib, mu, xintv, nslices, int(10 * ecount_full))
+                    if rc:
+                        raise RuntimeError("spslicer error %d" % rc)
+                ecount = ecount_full
+                out["mu"] = mu
         else:
             npts = 200
             rc, xd, yd, ecount = inst.landos(nvec, 40, npts, xintv)

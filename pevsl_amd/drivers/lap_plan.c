@@ -7,8 +7,13 @@
  * ChebLanNr, sort_double, FreePol, ParvecsFree, Finish) with one process
  * and one group, and prints per slice the computed and exact counts.
  *
+ * With -mlp <weights> the slices come from the learned slicer
+ * (pEVSL_SpslicerMlp); the full KPM run then judges them
+ * (pEVSL_SlicerJudge) and the driver falls back to pEVSL_SpslicerKpm when
+ * the largest full-KPM slice count exceeds the mean by more than 15%.
+ *
  * usage: lap_plan [-nx 20 -ny 20 -nz 20 -a 0.6 -b 1.2 -nslices 5
- *                  -mixed 0 -batch 0]
+ *                  -mixed 0 -batch 0 -mlp weights.bin]
  */
 #include <math.h>
 #include <stdio.h>
@@ -22,6 +27,11 @@ static int argi(int argc, char **argv, const char *k, int d) {
   for (int i = 1; i < argc - 1; i++)
     if (argv[i][0] == '-' && !strcmp(argv[i] + 1, k)) return atoi(argv[i + 1]);
   return d;
+}
+static const char *args(int argc, char **argv, const char *k) {
+  for (int i = 1; i < argc - 1; i++)
+    if (argv[i][0] == '-' && !strcmp(argv[i] + 1, k)) return argv[i + 1];
+  return NULL;
 }
 static double argd(int argc, char **argv, const char *k, double d) {
   for (int i = 1; i < argc - 1; i++)
@@ -89,13 +99,34 @@ int main(int argc, char **argv) {
   if (mixed) pEVSL_SetFilterPrecision(pevsl, PEVSL_PREC_FP32);
 
   double *mu = (double *)malloc((Mdeg + 1) * sizeof(double));
+  double *sli = (double *)malloc((nslices + 1) * sizeof(double));
+  const char *mlp_path = args(argc, argv, "mlp");
   double t0 = pEVSL_Wtime();
+  if (mlp_path) {
+    pevsl_Mlp *net;
+    double ecount_mlp;
+    if (pEVSL_MlpLoad(mlp_path, &net)) { printf("cannot load %s\n", mlp_path); return 1; }
+    ierr = pEVSL_SpslicerMlp(pevsl, net, xintv, nslices, sli, &ecount_mlp);
+    pEVSL_MlpFree(net);
+    if (ierr) { printf("SpslicerMlp error %d\n", ierr); return 1; }
+    printf("learned slicer: count %.1f (%.3f s)\n", ecount_mlp, pEVSL_Wtime() - t0);
+  }
   ierr = pEVSL_Kpmdos(pevsl, Mdeg, 1, nvec, xintv, 1, 0, MPI_COMM_NULL, mu, &ecount);
   if (ierr) { printf("kpmdos error %d\n", ierr); return 1; }
   int npts = 10 * ecount;
-  double *sli = (double *)malloc((nslices + 1) * sizeof(double));
-  ierr = pEVSL_SpslicerKpm(sli, mu, Mdeg, xintv, nslices, npts);
-  if (ierr) { printf("spslicer error %d\n", ierr); return 1; }
+  int use_kpm = 1;
+  if (mlp_path) {
+    double *cnt = (double *)malloc(nslices * sizeof(double)), imb;
+    pEVSL_SlicerJudge(mu, Mdeg, xintv, (double)n, nslices, sli, cnt, &imb);
+    use_kpm = imb > 0.15;
+    printf("judge (full KPM): largest slice %.1f%% above the mean -> %s\n", 100 * imb,
+           use_kpm ? "fall back to SpslicerKpm" : "accepted");
+    free(cnt);
+  }
+  if (use_kpm) {
+    ierr = pEVSL_SpslicerKpm(sli, mu, Mdeg, xintv, nslices, npts);
+    if (ierr) { printf("spslicer error %d\n", ierr); return 1; }
+  }
   printf("KPM count %.4f, slices:", ecount);
   for (int j = 0; j <= nslices; j++) printf(" %.6f", sli[j]);
   printf("  (%.3f s)\n", pEVSL_Wtime() - t0);

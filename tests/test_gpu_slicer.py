@@ -1,0 +1,173 @@
+"""GPU tests of the learned slicer: the network's HIP forward, backward and
+Adam kernels against the CPU oracle (oracle/slicer_oracle.py, FP64), the
+device training loop, the cheap-KPM features against the oracle KPM on the
+same libc rand() stream, and the slicer pipeline.
+
+Tolerances (FP32 kernels against the FP64 oracle): outputs 2e-5 relative to
+their scale, gradients 2e-4 of the largest gradient component, parameters
+after Adam steps 1e-5 absolute (each step moves a parameter by at most ~lr).
+"""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from oracle import slicer_oracle as so
+from pevsl_amd import driver, slicer
+
+pytestmark = pytest.mark.gpu
+
+SMALL = (5, 7, 2, 6)
+NSOFT = 4
+SLICER = (so.NIN, 256, 3, so.NBINS + 1)
+
+
+def banner(msg):
+    print("\n[gpu case] " + msg, flush=True)
+
+
+def _problem(shape, nsoft, batch, seed, wscale=0.5):
+    rng = np.random.default_rng(seed)
+    nin, nh, nl, nout = shape
+    nw = sum(o * i + o for (o, i) in so.layer_shapes(nin, nh, nl, nout))
+    w = rng.normal(0, 1, nw)
+    # He-like scale per layer so activations stay O(1) at width 256
+    off = 0
+    for (o, i) in so.layer_shapes(nin, nh, nl, nout):
+        w[off:off + o * i] *= wscale * np.sqrt(2.0 / i)
+        off += o * i + o
+    w = w.astype(np.float32).astype(np.float64)
+    X = rng.normal(0, 1, (batch, nin)).astype(np.float32).astype(np.float64)
+    Q = rng.normal(0, 0.3, (batch, nout)).astype(np.float32).astype(np.float64)
+    T = np.zeros((batch, nout))
+    t = rng.random((batch, nsoft)) ** 3
+    T[:, :nsoft] = t / t.sum(1, keepdims=True)
+    T[:, nsoft:] = rng.normal(0, 1, (batch, nout - nsoft))
+    T = T.astype(np.float32).astype(np.float64)
+    return w, X, Q, T
+
+
+@pytest.mark.parametrize("shape,nsoft,batch", [(SMALL, NSOFT, 3), (SLICER, so.NBINS, 16),
+                                               ((9, 300, 1, 70), 0, 70)])
+def test_forward_and_gradient_match_oracle(lib, shape, nsoft, batch):
+    banner("Mlp forward/grad shape=%s nsoft=%d batch=%d" % (shape, nsoft, batch))
+    w, X, Q, T = _problem(shape, nsoft, batch, seed=sum(shape))
+    m = slicer.Mlp(lib, *shape, nsoft=nsoft)
+    try:
+        m.set_weights(w)
+        Y = m.forward(X, Q)
+        z, _ = so.mlp_forward(w, shape, X, Q)
+        Yo, loss_o, _ = so.head(z, Q, None, nsoft)
+        scale = max(1.0, np.max(np.abs(Yo)))
+        assert np.max(np.abs(Y - Yo)) < 2e-5 * scale
+        loss, g = m.loss_grad(X, Q, T)
+        lo, go = so.mlp_loss_grad(w, shape, nsoft, X, Q, T)
+        assert abs(loss - lo) < 2e-5 * max(1.0, abs(lo))
+        assert np.max(np.abs(g - go)) < 2e-4 * np.max(np.abs(go))
+        # the prior enters additively: Q = 0 equals no prior
+        Y0 = m.forward(X, None)
+        Y0b = m.forward(X, np.zeros_like(Q))
+        assert np.array_equal(Y0, Y0b)
+    finally:
+        m.free()
+
+
+def test_train_step_matches_oracle_adam(lib):
+    banner("Mlp two Adam steps vs oracle")
+    shape = (6, 32, 2, 11)
+    w, X, Q, T = _problem(shape, 8, 12, seed=5)
+    m = slicer.Mlp(lib, *shape, nsoft=8)
+    try:
+        m.set_weights(w)
+        lr = 1e-3
+        wo, mo, vo = w.copy(), np.zeros_like(w), np.zeros_like(w)
+        for step in (1, 2):
+            m.train_step(X, Q, T, lr)
+            _, go = so.mlp_loss_grad(wo, shape, 8, X, Q, T)
+            wo, mo, vo = so.adam_step(wo, go, mo, vo, step, lr)
+        # parameters whose gradient is far above FP32 noise follow the oracle
+        # to rounding; near-zero gradients may flip sign in FP32, which Adam
+        # turns into a step of up to lr, so those are held to 2 lr
+        _, g0 = so.mlp_loss_grad(w, shape, 8, X, Q, T)
+        big = np.abs(g0) > 1e-4 * np.max(np.abs(g0))
+        d = np.abs(m.get_weights() - wo)
+        assert np.max(d[big]) < 1e-5 and np.max(d) < 2 * 2 * lr
+        # every parameter with a nonzero gradient moved
+        assert np.mean(np.abs(m.get_weights() - w) > 1e-6) > 0.5
+    finally:
+        m.free()
+
+
+def test_device_training_fits_a_teacher_and_save_load(lib, tmp_path):
+    banner("Mlp device training on a teacher network + save/load")
+    rng = np.random.default_rng(7)
+    shape = (8, 64, 2, 17)
+    teacher = slicer.Mlp(lib, *shape, nsoft=16, seed=99)
+    student = slicer.Mlp(lib, *shape, nsoft=16, seed=1)
+    try:
+        wt = teacher.get_weights()
+        # give the teacher a nonzero output layer
+        nw_out = 17 * 64 + 17
+        wt[-nw_out:] = rng.normal(0, 0.3, nw_out).astype(np.float32)
+        teacher.set_weights(wt)
+        X = rng.normal(0, 1, (2048, 8)).astype(np.float32)
+        T = teacher.forward(X)
+        losses = student.train(X, None, T, batch=128, epochs=60, lr=3e-3, seed=3)
+        assert np.all(np.isfinite(losses))
+        # cross-entropy floor = entropy of the teacher's distributions
+        ent = -np.mean(np.sum(T[:, :16] * np.log(np.maximum(T[:, :16], 1e-30)), 1))
+        assert losses[-1] - ent < 0.2 * (losses[0] - ent)
+        p = str(tmp_path / "m.bin")
+        student.save(p)
+        back = slicer.Mlp.load(p, lib)
+        try:
+            assert back.shape == student.shape and back.nsoft == 16
+            assert np.array_equal(back.get_weights(), student.get_weights())
+            assert np.array_equal(back.forward(X[:5]), student.forward(X[:5]))
+        finally:
+            back.free()
+    finally:
+        teacher.free()
+        student.free()
+
+
+def test_slicer_features_match_oracle_kpm(lib):
+    banner("SlicerFeatures 14x12x10 Laplacian vs oracle KPM (deg 20, 8 vectors)")
+    ia, ja, a = driver.laplacian3d(14, 12, 10)
+    M = driver.Matrix(lib, ia, ja, a)
+    inst = driver.Instance(lib, M)  # pEVSL_Start seeds srand(0)
+    intv = [0.6, 1.2, 0.0, 12.0]
+    try:
+        f, q = slicer.features(inst, intv)
+    finally:
+        inst.close()
+        M.free()
+    orc.srand(0)
+    rc, mu, _ = orc.kpmdos(orc.Csr(ia, ja, a), 20, 1, 8, intv)
+    assert rc == 0
+    fo, qo = so.features(mu, len(ia) - 1, intv)
+    assert np.max(np.abs(f - fo)) < 1e-5
+    assert np.max(np.abs(q - qo)) < 1e-4
+
+
+def test_untrained_network_reproduces_cheap_kpm_slices(lib):
+    banner("SpslicerMlp with a zero output layer = cuts of the cheap KPM DOS")
+    ia, ja, a = driver.laplacian3d(16, 16, 16)
+    M = driver.Matrix(lib, ia, ja, a)
+    intv = [0.6, 1.2, 0.0, 12.0]
+    m = slicer.Mlp(lib)  # output layer starts at zero
+    inst = driver.Instance(lib, M)
+    try:
+        sli, ecnt = slicer.spslicer_mlp(inst, m, intv, 5)
+    finally:
+        inst.close()
+        m.free()
+        M.free()
+    orc.srand(0)
+    rc, mu, oecnt = orc.kpmdos(orc.Csr(ia, ja, a), 20, 1, 8, intv)
+    mo, fr = so.bin_masses(mu, intv, so.NBINS)
+    so_sli = so.cuts_from_bins(mo, 0.6, 1.2, 5)
+    assert abs(sli[0] - 0.6) < 1e-15 and abs(sli[-1] - 1.2) < 1e-15
+    # softmax of log masses in FP32: cuts agree to FP32 rounding
+    assert np.max(np.abs(sli - so_sli)) < 1e-5
+    assert abs(ecnt / (fr * 16 ** 3) - 1) < 1e-5
