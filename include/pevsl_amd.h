@@ -194,10 +194,11 @@ int pEVSL_MlpTrainStep(pevsl_Mlp *mlp, const float *X, const float *Q, const flo
                        int batch, float lr, float *loss);
 /* `epochs` shuffled passes over a device-resident training set, minibatches
  * of `batch`, Adam with cosine decay of lr to 2%; losses[e] = mean loss of
- * epoch e (may be NULL). */
+ * epoch e (may be NULL).  verbose > 0 prints the loss every `verbose`
+ * epochs to stdout. */
 int pEVSL_MlpTrain(pevsl_Mlp *mlp, const float *X, const float *Q, const float *T,
                    int nsamples, int batch, int epochs, float lr, unsigned long long seed,
-                   float *losses);
+                   float *losses, int verbose);
 int pEVSL_MlpSave(pevsl_Mlp *mlp, const char *path);
 int pEVSL_MlpLoad(const char *path, pevsl_Mlp **mlp);
 

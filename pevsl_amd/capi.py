@@ -253,7 +253,8 @@ SIGNATURES = {
     "pEVSL_MlpTrainStep": (C.c_int, [C.c_void_p, c_float_p, c_float_p, c_float_p, C.c_int,
                                      C.c_float, c_float_p]),
     "pEVSL_MlpTrain": (C.c_int, [C.c_void_p, c_float_p, c_float_p, c_float_p, C.c_int,
-                                 C.c_int, C.c_int, C.c_float, C.c_ulonglong, c_float_p]),
+                                 C.c_int, C.c_int, C.c_float, C.c_ulonglong, c_float_p,
+                                 C.c_int]),
     "pEVSL_MlpSave": (C.c_int, [C.c_void_p, C.c_char_p]),
     "pEVSL_MlpLoad": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "pEVSL_KpmBinMasses": (C.c_int, [c_double_p, C.c_int, c_double_p, C.c_int, c_double_p,

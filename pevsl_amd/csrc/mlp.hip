@@ -451,7 +451,8 @@ int pEVSL_MlpTrainStep(pevsl_Mlp *mp, const float *X, const float *Q, const floa
 }
 
 int pEVSL_MlpTrain(pevsl_Mlp *mp, const float *X, const float *Q, const float *T, int nsamples,
-                   int batch, int epochs, float lr, unsigned long long seed, float *losses) {
+                   int batch, int epochs, float lr, unsigned long long seed, float *losses,
+                   int verbose) {
   Mlp *m = mlp_of(mp);
   if (!m || !X || !T || nsamples < 1 || batch < 1 || epochs < 1) return -1;
   batch = std::min(batch, nsamples);
@@ -494,6 +495,13 @@ int pEVSL_MlpTrain(pevsl_Mlp *mp, const float *X, const float *Q, const float *T
                          1.f / ((float)bs * steps_per_epoch), dl.p + e);
       const double f = 0.02 + 0.98 * 0.5 * (1.0 + std::cos(M_PI * (double)it / total));
       m->adam((float)(lr * f));
+    }
+    if (verbose > 0 && ((e + 1) % verbose == 0 || e + 1 == epochs)) {
+      float le = 0.f;
+      PV_HIP(hipMemcpyAsync(&le, dl.p + e, sizeof(float), hipMemcpyDeviceToHost, m->s));
+      PV_HIP(hipStreamSynchronize(m->s));
+      fprintf(stdout, "MlpTrain: epoch %d/%d mean loss %.6f\n", e + 1, epochs, le);
+      fflush(stdout);
     }
   }
   if (losses)

@@ -268,7 +268,7 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
                     keep_vectors=False, rand_mode=capi.PEVSL_RAND_LIBC, seed=0,
                     sli_override=None, block_width=0, log=None, verbose=False,
                     maxit_cap=None, lanes=0, ritz_refine=-1, mixed_tol=None,
-                    block_rowkernel=-1, mlp=None, judge=True, judge_tol=0.15):
+                    block_rowkernel=-1, mlp=None, judge=True, judge_tol=0.15, wave=None):
     """End-to-end run of the TESTS/Lap/LapPLanN.c flow on the GPU.  Returns a
     dict with the DOS, slices, per-slice filters, eigenvalues and timings.
 
@@ -276,7 +276,11 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
     3: learned slicer (`mlp`: a slicer.Mlp or a weights path).  With
     `judge`, dostype 3 also runs the full KPM, and when the full-KPM counts of
     the proposed slices are more than `judge_tol` above their mean it falls
-    back to SpslicerKpm on the full moments."""
+    back to SpslicerKpm on the full moments.
+
+    With `batch`, slices are solved `wave` at a time (all at once when None)
+    as the columns of one pEVSL_ChebLanNrBatch call, so that the Lanczos
+    bases of one wave fit in HBM."""
     lib = capi.load()
     t0 = time.time()
     ia, ja, av = laplacian3d(nx, ny, nz)
@@ -350,7 +354,11 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
         out["t_findpol"] = time.time() - t2
         res_slices = []
         t3 = time.time()
-        groups = [list(range(len(todo)))] if batch else [[i] for i in range(len(todo))]
+        if batch:
+            w_ = wave or len(todo)
+            groups = [list(range(i, min(i + w_, len(todo)))) for i in range(0, len(todo), w_)]
+        else:
+            groups = [[i] for i in range(len(todo))]
         for g in groups:
             rc, outs = cheblan_batch(inst, [intvs[i] for i in g], [mlan] * len(g), tol,
                                      vinit, [pols[i] for i in g], keep_vectors,

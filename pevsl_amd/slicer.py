@@ -136,13 +136,13 @@ class Mlp:
             raise RuntimeError("pEVSL_MlpTrainStep failed: %d" % rc)
         return loss.value
 
-    def train(self, X, Q, T, batch=256, epochs=100, lr=1e-3, seed=0):
+    def train(self, X, Q, T, batch=256, epochs=100, lr=1e-3, seed=0, verbose=0):
         X = _f32(X)
         Qa = None if Q is None else _f32(Q)
         T = _f32(T)
         losses = np.zeros(epochs, np.float32)
         rc = self.lib.pEVSL_MlpTrain(self.h, _ptr(X), _ptr(Qa), _ptr(T), X.shape[0], batch,
-                                     epochs, lr, seed, _ptr(losses))
+                                     epochs, lr, seed, _ptr(losses), verbose)
         if rc:
             raise RuntimeError("pEVSL_MlpTrain failed: %d" % rc)
         return losses
@@ -302,7 +302,7 @@ def load_moments(path):
              "mu_cheap": z["mu_cheap"][i]} for i in range(len(z["n"]))]
 
 
-def make_pairs(lib, mats, nint, seed):
+def make_pairs(lib, mats, nint, seed, log=print):
     """Training pairs (features, prior, target) for `nint` random intervals
     per matrix; targets are full-KPM bin masses (see module doc)."""
     rng = np.random.default_rng(seed + 7)
@@ -322,6 +322,9 @@ def make_pairs(lib, mats, nint, seed):
             T.append(np.concatenate([ms / ms.sum(), [math.log(fr)]]))
             meta.append((j, m["dims"][0], m["dims"][1], m["dims"][2], m["w"], lo, hi, lmin,
                          lmax))
+        if log and ((j + 1) % 1000 == 0 or j + 1 == len(mats)):
+            log("pairs: %d/%d matrices, %d samples" % (j + 1, len(mats), len(X)))
+            sys.stdout.flush()
     return (np.array(X, np.float32), np.array(Q, np.float32), np.array(T, np.float32),
             np.array(meta))
 
@@ -397,7 +400,8 @@ def main(argv=None):
     va = ~tr
     mlp = Mlp(lib, seed=args.seed)
     t1 = time.time()
-    losses = mlp.train(X[tr], Q[tr], T[tr], args.batch, args.epochs, args.lr, args.seed)
+    losses = mlp.train(X[tr], Q[tr], T[tr], args.batch, args.epochs, args.lr, args.seed,
+                       verbose=10)
     t_train = time.time() - t1
     print("training: %d samples, %d epochs, loss %.5f -> %.5f, %.1f s" % (
         int(tr.sum()), args.epochs, losses[0], losses[-1], t_train))

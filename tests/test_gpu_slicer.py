@@ -8,6 +8,8 @@ their scale, gradients 2e-4 of the largest gradient component, parameters
 after Adam steps 1e-5 absolute (each step moves a parameter by at most ~lr).
 """
 
+import os
+
 import numpy as np
 import pytest
 
@@ -171,3 +173,133 @@ def test_untrained_network_reproduces_cheap_kpm_slices(lib):
     # softmax of log masses in FP32: cuts agree to FP32 rounding
     assert np.max(np.abs(sli - so_sli)) < 1e-5
     assert abs(ecnt / (fr * 16 ** 3) - 1) < 1e-5
+
+
+# ------------------------------------------------------- the trained network
+def _slicings(lib, net, ia, ja, a, intv, nslices, full=(300, 60)):
+    """Slices of the learned slicer, of SpslicerKpm on the same cheap KPM run
+    (the method without the network at equal cost) and of SpslicerKpm on the
+    full KPM run (the reference driver's settings); full-KPM moments too."""
+    M = driver.Matrix(lib, ia, ja, a)
+    out = {}
+    try:
+        inst = driver.Instance(lib, M)  # each instance restarts srand(0)
+        try:
+            out["net"], out["net_count"] = slicer.spslicer_mlp(inst, net, intv, nslices)
+        finally:
+            inst.close()
+        inst = driver.Instance(lib, M)
+        try:
+            rc, mu, ecnt = inst.kpmdos(20, 1, 8, intv)
+            assert rc == 0
+            out["cheap"] = slicer.kpm_slices(lib, mu, intv, nslices, ecnt)
+            out["cheap_count"] = ecnt
+        finally:
+            inst.close()
+        inst = driver.Instance(lib, M)
+        try:
+            rc, mu, ecnt = inst.kpmdos(full[0], 1, full[1], intv)
+            assert rc == 0
+            out["full"] = slicer.kpm_slices(lib, mu, intv, nslices, ecnt)
+            out["full_count"] = ecnt
+            out["mu_full"] = mu
+        finally:
+            inst.close()
+    finally:
+        M.free()
+    return out
+
+
+def _imb(counts):
+    counts = np.asarray(counts, float)
+    return counts.max() / counts.mean() - 1.0
+
+
+@pytest.fixture(scope="module")
+def trained(lib):
+    if not os.path.exists(slicer.DEFAULT_WEIGHTS):
+        pytest.fail("trained weights missing: %s" % slicer.DEFAULT_WEIGHTS)
+    net = slicer.Mlp.load(slicer.DEFAULT_WEIGHTS, lib)
+    yield net
+    net.free()
+
+
+LAPLACIANS = [  # (nx, ny, nz, a, b, nslices): exact counts from the closed form
+    (100, 100, 100, 0.6, 1.2, 20),
+    (80, 70, 60, 2.0, 3.5, 10),
+    (50, 50, 50, 5.5, 6.5, 10),
+    (300, 300, 1, 1.0, 2.0, 10),
+    (90, 90, 90, 9.0, 10.5, 10),
+]
+
+
+def test_trained_slicer_beats_cheap_kpm_on_exact_counts(lib, trained):
+    """Gain of the network over the method without it at the same KPM cost
+    (degree 20, 8 vectors = 160 SpMVs), judged by the exact Laplacian
+    spectrum; full KPM (degree 300, 60 vectors = 18,000 SpMVs) for scale."""
+    rows = []
+    for (nx, ny, nz, a, b, ns) in LAPLACIANS:
+        banner("learned slicer vs KPM, Laplacian %dx%dx%d [%g, %g] %d slices" % (
+            nx, ny, nz, a, b, ns))
+        ia, ja, av = driver.laplacian3d(nx, ny, nz)
+        lmax = 12.0 if nz > 1 else 8.0
+        s = _slicings(lib, trained, ia, ja, av, [a, b, 0.0, lmax], ns)
+        ev = driver.exact_eig_lap3(nx, ny, nz, a, b)
+        r = {"case": (nx, ny, nz, a, b, ns), "exact_count": len(ev)}
+        for k in ("net", "cheap", "full"):
+            assert np.all(np.diff(s[k]) > 0)
+            r[k] = _imb(np.histogram(ev, bins=s[k])[0])
+            r[k + "_cnt_err"] = abs(s[k + "_count"] / len(ev) - 1)
+        rows.append(r)
+        print("  imbalance net %.4f cheap %.4f full %.4f | count error net %.4f cheap %.4f "
+              "full %.4f" % (r["net"], r["cheap"], r["full"], r["net_cnt_err"],
+                             r["cheap_cnt_err"], r["full_cnt_err"]), flush=True)
+    net = np.mean([r["net"] for r in rows])
+    cheap = np.mean([r["cheap"] for r in rows])
+    assert net < cheap, (net, cheap)
+    assert np.mean([r["net_cnt_err"] for r in rows]) < np.mean([r["cheap_cnt_err"]
+                                                                 for r in rows])
+
+
+def test_trained_slicer_beats_cheap_kpm_on_unseen_potentials(lib, trained):
+    """Matrices with random diagonal potentials from seeds outside the
+    training set; full KPM judges the slices (pEVSL_SlicerJudge)."""
+    cases = [((40, 40, 40), 1.0, 4321, (3.0, 5.0), 10),
+             ((250, 250, 1), 0.5, 4322, (1.5, 3.0), 10),
+             ((30, 50, 20), 1.7, 4323, (7.0, 9.5), 8),
+             ((64, 64, 64), 0.3, 4324, (0.8, 1.6), 16)]
+    net_i, cheap_i = [], []
+    for (dims, w, seed, (a, b), ns) in cases:
+        banner("learned slicer vs KPM, potential w=%.1f dims %s [%g, %g]" % (w, dims, a, b))
+        ia, ja, av, (lmin, lmax) = slicer.lap_potential(*dims, w, seed)
+        intv = [a, b, lmin, lmax]
+        s = _slicings(lib, trained, ia, ja, av, intv, ns)
+        n = dims[0] * dims[1] * dims[2]
+        _, jn = slicer.judge(lib, s["mu_full"], intv, n, s["net"])
+        _, jc = slicer.judge(lib, s["mu_full"], intv, n, s["cheap"])
+        net_i.append(jn)
+        cheap_i.append(jc)
+        print("  full-KPM judge: imbalance net %.4f cheap %.4f" % (jn, jc), flush=True)
+    assert np.mean(net_i) < np.mean(cheap_i)
+
+
+def test_mlp_sliced_solve_counts_match_exact_and_oracle(lib, trained):
+    """End to end on the 20^3 Laplacian: learned slices, FindPol, batched
+    ChebLanNr (FP64); every slice finds the exact closed-form count, and the
+    oracle's ChebLanNr on the same slices finds the same counts."""
+    banner("MLP-sliced batched ChebLanNr 20^3 [0.6, 1.2] 5 slices")
+    res = driver.lap_slicing_run(20, 20, 20, 0.6, 1.2, 5, dostype=3, mlp=trained,
+                                 judge=True, batch=True)
+    assert res["judge"]["imbalance"] < 0.5
+    sl = res["slices"]
+    for d in sl:
+        assert d["nev"] == d["nev_exact"], d["slice"]
+        if d["nev"]:
+            assert d["max_rel_err"] < 1e-10
+    # the oracle's ChebLanNr on two of the learned slices (its own libc
+    # stream for vinit; -nev 20 gives the same mlan = 300)
+    assert sl[0]["mlan"] == 300
+    for d in sl[:2]:
+        o = orc.run_driver(["-nx", 20, "-ny", 20, "-nz", 20, "-a", repr(d["intv"][0]),
+                            "-b", repr(d["intv"][1]), "-dostype", 2, "-nev", 20], timeout=120)
+        assert o["slices"][0]["nev"] == d["nev"], (o["slices"][0]["nev"], d["nev"])
