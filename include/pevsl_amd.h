@@ -54,6 +54,15 @@ typedef struct {
 #define PEVSL_PREC_FP32 1
 int pEVSL_SetFilterPrecision(pevsl_Data *pevsl, int prec);
 
+/* Storage of the Lanczos basis on the FP32-filter path: PEVSL_PREC_FP64
+ * (default) or PEVSL_PREC_FP32.  With FP32 storage the reorthogonalization
+ * reads half the bytes; its dots and updates still accumulate in FP64, the
+ * tridiagonal problem, the Rayleigh-Ritz refinement and the residuals stay
+ * FP64, and the basis is orthonormal to FP32 rounding.  Ignored when the
+ * filter runs in FP64 or A is a host callback.  No counterpart in the
+ * reference. */
+int pEVSL_SetBasisPrecision(pevsl_Data *pevsl, int prec);
+
 /* Residual threshold used instead of ChebLanNr's tol when the filter runs in
  * FP32 (default 1e-5): Ritz vectors built from an FP32-filtered Krylov basis
  * carry FP32-level errors (~1e-6 relative), while their FP64 Rayleigh

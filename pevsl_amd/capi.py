@@ -219,6 +219,7 @@ SIGNATURES = {
     "pEVSL_SetFilterPrecision": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetMixedTol": (C.c_int, [DataP, C.c_double]),
     "pEVSL_SetRitzRefine": (C.c_int, [DataP, C.c_int]),
+    "pEVSL_SetBasisPrecision": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetRandMode": (C.c_int, [DataP, C.c_int, C.c_ulonglong]),
     "pEVSL_SetKeepVectors": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetBlockKernel": (C.c_int, [DataP, C.c_int]),

@@ -222,6 +222,12 @@ int pEVSL_SetFilterPrecision(pevsl_Data *p, int prec) {
   return 0;
 }
 
+int pEVSL_SetBasisPrecision(pevsl_Data *p, int prec) {
+  if (prec != PEVSL_PREC_FP64 && prec != PEVSL_PREC_FP32) return -1;
+  ctx_of(p).basis_prec = prec;
+  return 0;
+}
+
 int pEVSL_SetRitzRefine(pevsl_Data *p, int mode) {
   if (mode < -1 || mode > 1) return -1;
   ctx_of(p).ritz_refine = mode;

@@ -329,7 +329,7 @@ int pEVSL_LanDosG(pevsl_Data *pevsl, const int nvec, int msteps, int npts, doubl
   PV_HIP(hipMemsetAsync(V.p, 0, sizeof(double) * L.ld * ((size_t)1 << shift) * nvec, s));
   std::vector<double *> vt((size_t)nvec);
   for (int v = 0; v < nvec; v++) vt[(size_t)v] = V.p + L.ld * ((size_t)1 << shift) * v;
-  L.vtab = vtab.ensure((size_t)nvec);
+  L.vtab = reinterpret_cast<void **>(vtab.ensure((size_t)nvec));
   PV_HIP(hipMemcpyAsync(L.vtab, vt.data(), sizeof(double *) * nvec, hipMemcpyHostToDevice, s));
   double *scp = sc.ensure((size_t)4 * nvec);
   L.alpha = scp;

@@ -75,6 +75,7 @@ struct Ctx {
   MVFunc host_mv = nullptr;        // set by SetAMatvec (host callback)
   void *host_mv_data = nullptr;
   int filter_prec = PEVSL_PREC_FP64;
+  int basis_prec = PEVSL_PREC_FP64;  // Lanczos basis storage (FP32 only with an FP32 filter)
   double mixed_tol = 1e-5;         // residual threshold of the FP32-filter path
   int ritz_refine = -1;            // FP64 Rayleigh-Ritz refinement: 1 on, 0 off, -1 FP32 only
   int rand_mode = PEVSL_RAND_LIBC;
@@ -137,5 +138,7 @@ void rand_vectors(Ctx &c, int n, int nvec, double *d_out, size_t ld);
 // (v, y in HBM).  Work vectors come from the context.
 void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
                      double *dy);
+// The same for FP32 columns (FP32 filter arithmetic; device matrix only).
+void cheb_filter_col32(Ctx &c, const pevsl_Polparams *pol, const float *dv, float *dy);
 
 }  // namespace pevsl

@@ -68,6 +68,64 @@ __device__ __forceinline__ double2 ld_basis(const double2 *p) {
   return *p;
 }
 
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 ld_basis_nt(const float4 *p) {
+  const fvec4 v = __builtin_nontemporal_load(reinterpret_cast<const fvec4 *>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// 16-byte row groups of the Lanczos basis: R rows of type TV per load, kept
+// in the loaded type and widened to FP64 where they are used.  Launch shapes
+// per element type keep the rows per block (and so the grids) of the two
+// storage types equal.
+template <typename TV>
+struct Row16;
+template <>
+struct Row16<double> {
+  using vec = double2;
+  static constexpr int R = 2;
+  static constexpr int kTiters = 16;   // GEMV-T row groups per thread
+  static constexpr int kGnPairs = 2;   // GEMV-N row groups per thread
+  static constexpr int kGnUnroll = 8;  // GEMV-N columns per load batch
+  template <bool NT>
+  __device__ static __forceinline__ vec ld(const vec *p) { return ld_basis<NT>(p); }
+  __device__ static __forceinline__ double get(const vec &v, int r) { return r ? v.y : v.x; }
+  __device__ static __forceinline__ vec zero() { return make_double2(0.0, 0.0); }
+  __device__ static __forceinline__ void store(vec *p, const double (&o)[R]) {
+    *p = make_double2(o[0], o[1]);
+  }
+};
+template <>
+struct Row16<float> {
+  using vec = float4;
+  static constexpr int R = 4;
+  static constexpr int kTiters = 8;
+  static constexpr int kGnPairs = 1;
+  static constexpr int kGnUnroll = 16;
+  template <bool NT>
+  __device__ static __forceinline__ vec ld(const vec *p) { return NT ? ld_basis_nt(p) : *p; }
+  __device__ static __forceinline__ double get(const vec &v, int r) {
+    return r == 0 ? v.x : (r == 1 ? v.y : (r == 2 ? v.z : v.w));
+  }
+  __device__ static __forceinline__ vec zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+  __device__ static __forceinline__ void store(vec *p, const double (&o)[R]) {
+    *p = make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
+  }
+};
+
+// Row-block index of this workgroup such that each XCD works on one
+// contiguous range of row blocks.  Workgroups are dispatched to the 8 XCDs
+// round-robin (workgroup i to XCD i mod 8), and each XCD has its own L2: with
+// contiguous ranges the neighbour rows that a stencil-like matrix gathers
+// (+-1, +-nx, +-nx*ny) mostly sit in the same L2.  A bijection of [0, nb).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int bid, int nb) {
+  const int q = nb / kXcds, r = nb % kXcds;
+  const int x = bid % kXcds, j = bid / kXcds;
+  return x * q + min(x, r) + j;
+}
+
 inline int grid_for(size_t work, int threads = kThreads) {
   return (int)((work + threads - 1) / threads);
 }
@@ -149,7 +207,8 @@ k_cheb_stream(const int *__restrict__ rowblk, const int *__restrict__ ia,
               int first) {
 #pragma clang fp contract(off)
   __shared__ T prod[kStreamNnz];
-  const int r0 = rowblk[blockIdx.x], r1 = rowblk[blockIdx.x + 1];
+  const int rb = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = rowblk[rb], r1 = rowblk[rb + 1];
   const int j0 = ia[r0], nz = ia[r1] - j0;
   T acc = 0;
   int r = r0 + threadIdx.x;
@@ -256,7 +315,8 @@ k_cheb_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
 #pragma clang fp contract(off)
   __shared__ T as[kStreamNnz];
   __shared__ int js[kStreamNnz];
-  const int r0 = rowblk[blockIdx.x], r1 = rowblk[blockIdx.x + 1];
+  const int rb = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = rowblk[rb], r1 = rowblk[rb + 1];
   const int j0 = ia[r0], nz = ia[r1] - j0;
   const bool staged = nz <= kStreamNnz;
   if (staged) {
@@ -396,6 +456,21 @@ k_dot_part(int n, const double *__restrict__ x, const double *__restrict__ y,
   block_sum_store(acc, part + blockIdx.x);
 }
 
+__global__ void __launch_bounds__(kThreads)
+k_dot_part_f32(int n, const float *__restrict__ x, const float *__restrict__ y,
+               double *__restrict__ part) {
+  double acc = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < (size_t)n;
+       i += (size_t)gridDim.x * kThreads)
+    acc += (double)x[i] * (double)y[i];
+  block_sum_store(acc, part + blockIdx.x);
+}
+
+__global__ void k_scale_f32(int n, float *__restrict__ x, double hs) {
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < (size_t)n) x[i] = (float)((double)x[i] * hs);
+}
+
 __global__ void k_scale(int n, double *__restrict__ x,
                         const double *__restrict__ inv_dev, double hs) {
   const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
@@ -422,8 +497,8 @@ __global__ void k_zero(size_t n, double *__restrict__ x) {
   if (i < n) x[i] = 0.0;
 }
 
-template <typename T>
-__global__ void k_pack(int n, int B, int ncol, const double *const *cols,
+template <typename T, typename TV>
+__global__ void k_pack(int n, int B, int ncol, const TV *const *cols,
                        T *__restrict__ blk) {
   const size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x;
   const size_t row = g / B;
@@ -432,26 +507,27 @@ __global__ void k_pack(int n, int B, int ncol, const double *const *cols,
   blk[g] = c < ncol ? (T)cols[c][row] : (T)0;
 }
 
-template <typename T>
+template <typename T, typename TV>
 __global__ void k_unpack(int n, int B, int ncol, const T *__restrict__ blk,
-                         double *const *cols) {
+                         TV *const *cols) {
   const size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x;
   const size_t row = g / B;
   const int c = (int)(g % B);
   if (row >= (size_t)n || c >= ncol) return;
-  cols[c][row] = (double)blk[g];
+  cols[c][row] = (TV)blk[g];
 }
 
 // ---------------------------------------------------------------------------
 // Lanczos (batched over slices; blockIdx.y = slice)
 // ---------------------------------------------------------------------------
 
+template <typename TV>
 struct LanArgs {
   int n;
   size_t ld;
   int chunk_shift;
   int maxchunks;
-  double **vtab;
+  TV **vtab;
   double *alpha, *beta, *nrm2a, *nrm2b;
   int *pass2, *active;
   double *h;
@@ -459,13 +535,14 @@ struct LanArgs {
   double *part;
 };
 
-LanArgs make_args(const LanBatch &L) {
-  LanArgs a;
+template <typename TV>
+LanArgs<TV> make_args(const LanBatch &L) {
+  LanArgs<TV> a;
   a.n = L.n;
   a.ld = L.ld;
   a.chunk_shift = L.chunk_shift;
   a.maxchunks = L.maxchunks;
-  a.vtab = L.vtab;
+  a.vtab = reinterpret_cast<TV **>(L.vtab);
   a.alpha = L.alpha;
   a.beta = L.beta;
   a.nrm2a = L.nrm2a;
@@ -478,7 +555,8 @@ LanArgs make_args(const LanBatch &L) {
   return a;
 }
 
-__device__ __forceinline__ double *lan_col(const LanArgs &L, int s, int j) {
+template <typename TV>
+__device__ __forceinline__ TV *lan_col(const LanArgs<TV> &L, int s, int j) {
   const int mask = (1 << L.chunk_shift) - 1;
   return L.vtab[(size_t)s * L.maxchunks + (j >> L.chunk_shift)] +
          (size_t)(j & mask) * L.ld;
@@ -491,50 +569,53 @@ inline int lan_vec_blocks(int n) {
 }
 
 // znew -= beta zold (k > 0); partial <z, znew>
+template <typename TV>
 __global__ void __launch_bounds__(kThreads)
-k_lan_alpha(LanArgs L, int k) {
+k_lan_alpha(LanArgs<TV> L, int k) {
 #pragma clang fp contract(off)
   const int s = blockIdx.y;
   if (!L.active[s]) return;
-  double *znew = lan_col(L, s, k + 1);
-  const double *z = lan_col(L, s, k);
-  const double *zold = k > 0 ? lan_col(L, s, k - 1) : nullptr;
+  TV *znew = lan_col(L, s, k + 1);
+  const TV *z = lan_col(L, s, k);
+  const TV *zold = k > 0 ? lan_col(L, s, k - 1) : nullptr;
   const double mb = -L.beta[s];
   double acc = 0.0;
   for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < (size_t)L.n;
        i += (size_t)gridDim.x * kThreads) {
     double w = znew[i];
     if (zold) {
-      w = w + mb * zold[i];
-      znew[i] = w;
+      w = w + mb * (double)zold[i];
+      znew[i] = (TV)w;
     }
-    acc += z[i] * w;
+    acc += (double)z[i] * w;
   }
   block_sum_store(acc, L.part + (size_t)s * gridDim.x + blockIdx.x);
 }
 
 // znew -= alpha z; partial |znew|^2
+template <typename TV>
 __global__ void __launch_bounds__(kThreads)
-k_lan_sub_alpha(LanArgs L, int k) {
+k_lan_sub_alpha(LanArgs<TV> L, int k) {
 #pragma clang fp contract(off)
   const int s = blockIdx.y;
   if (!L.active[s]) return;
-  double *znew = lan_col(L, s, k + 1);
-  const double *z = lan_col(L, s, k);
+  TV *znew = lan_col(L, s, k + 1);
+  const TV *z = lan_col(L, s, k);
   const double ma = -L.alpha[s];
   double acc = 0.0;
   for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < (size_t)L.n;
        i += (size_t)gridDim.x * kThreads) {
-    const double w = znew[i] + ma * z[i];
+    const TV w = (TV)((double)znew[i] + ma * (double)z[i]);
     znew[i] = w;
-    acc += w * w;
+    acc += (double)w * (double)w;
   }
   block_sum_store(acc, L.part + (size_t)s * gridDim.x + blockIdx.x);
 }
 
 // per-slice sums of the vector partials into one of the scalar arrays
 // which: 0 alpha, 1 nrm2a, 2 nrm2b (pass 0, sets pass2), 3 nrm2b (pass 1)
-__global__ void k_lan_reduce(LanArgs L, int nblk, int which) {
+template <typename TV>
+__global__ void k_lan_reduce(LanArgs<TV> L, int nblk, int which) {
   const int s = blockIdx.x;
   if (!L.active[s]) return;
   if (which == 3 && !L.pass2[s]) return;
@@ -558,17 +639,22 @@ __global__ void k_lan_reduce(LanArgs L, int nblk, int which) {
 
 // GEMV-T partials: part[(s*hcap + j)*nrb + rb] = sum over row block rb of
 // V_s(:,j) * znew_s, for the kTcols columns of this block.  Each lane keeps
-// one accumulator per column and streams row pairs (16-byte loads, one
-// independent load per column per iteration); the cross-lane reduction runs
-// once per column at the end.  Rows in [n, ld) are zero in every basis
-// column, so whole row pairs up to ld are read without masks.
+// one accumulator per column and streams 16-byte row groups (2 FP64 or 4
+// FP32 rows, one independent load per column per iteration); the cross-lane
+// reduction runs once per column at the end.  Rows in [n, ld) are zero in
+// every basis column, so whole row groups up to ld are read without masks.
 constexpr int kTcols = 32;               // columns per block
-constexpr int kTiters = 16;              // row pairs per thread
-constexpr int kGtRows = kThreads * 2 * kTiters;  // rows per block (8192)
+template <typename TV>
+constexpr int gt_rows() {  // rows per block: 8192 for both storage types
+  return kThreads * Row16<TV>::R * Row16<TV>::kTiters;
+}
 
-template <bool NT>
+template <bool NT, typename TV>
 __global__ void __launch_bounds__(kThreads)
-k_lan_gemvt(LanArgs L, int k, int pass, int nrb) {
+k_lan_gemvt(LanArgs<TV> L, int k, int pass, int nrb) {
+  using RV = Row16<TV>;
+  using vec = typename RV::vec;
+  constexpr int R = RV::R;
   const int s = blockIdx.z;
   if (!L.active[s]) return;
   if (pass && !L.pass2[s]) return;
@@ -577,42 +663,45 @@ k_lan_gemvt(LanArgs L, int k, int pass, int nrb) {
   if (j0 >= ncols) return;
   const int jn = min(kTcols, ncols - j0);
   const int rb = blockIdx.x;
-  const double2 *z = reinterpret_cast<const double2 *>(lan_col(L, s, k + 1));
+  const vec *z = reinterpret_cast<const vec *>(lan_col(L, s, k + 1));
   // the kTcols columns of a block lie in one basis chunk (chunks hold a
   // power of two >= kTcols columns), kTcols-aligned: column c is at
   // base + c*ld
-  const double2 *base = reinterpret_cast<const double2 *>(lan_col(L, s, j0));
-  const size_t cstride = L.ld / 2;
+  const vec *base = reinterpret_cast<const vec *>(lan_col(L, s, j0));
+  const size_t cstride = L.ld / R;
   double acc[kTcols];
 #pragma unroll
   for (int c = 0; c < kTcols; c++) acc[c] = 0.0;
-  const size_t npair = L.ld / 2;
-  const size_t p0 = (size_t)rb * (kGtRows / 2) + threadIdx.x;
-  if (jn == kTcols) {
-    for (int i = 0; i < kTiters; i++) {
+  const size_t ngrp = L.ld / R;
+  const size_t p0 = (size_t)rb * (gt_rows<TV>() / R) + threadIdx.x;
+  // full column groups load without guards; the last group of a basis
+  // guards its columns
+  auto sweep = [&](auto full) {
+    for (int i = 0; i < RV::kTiters; i++) {
       const size_t p = p0 + (size_t)i * kThreads;
-      if (p >= npair) break;
-      const double2 zz = z[p];
+      if (p >= ngrp) break;
+      const vec zv = z[p];
+      double zz[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) zz[r] = RV::get(zv, r);
+      vec v[kTcols];
+#pragma unroll
+      for (int c = 0; c < kTcols; c++)
+        v[c] = (decltype(full)::value || c < jn) ? RV::template ld<NT>(base + c * cstride + p)
+                                                 : RV::zero();
 #pragma unroll
       for (int c = 0; c < kTcols; c++) {
-        const double2 v = ld_basis<NT>(base + c * cstride + p);
-        acc[c] += v.x * zz.x + v.y * zz.y;
-      }
-    }
-  } else {
-    for (int i = 0; i < kTiters; i++) {
-      const size_t p = p0 + (size_t)i * kThreads;
-      if (p >= npair) break;
-      const double2 zz = z[p];
+        double t = RV::get(v[c], 0) * zz[0];
 #pragma unroll
-      for (int c = 0; c < kTcols; c++) {
-        if (c < jn) {
-          const double2 v = ld_basis<NT>(base + c * cstride + p);
-          acc[c] += v.x * zz.x + v.y * zz.y;
-        }
+        for (int r = 1; r < R; r++) t += RV::get(v[c], r) * zz[r];
+        acc[c] += t;
       }
     }
-  }
+  };
+  if (jn == kTcols)
+    sweep(std::true_type{});
+  else
+    sweep(std::false_type{});
   __shared__ double red[kThreads / 64][kTcols];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -630,8 +719,9 @@ k_lan_gemvt(LanArgs L, int k, int pass, int nrb) {
 }
 
 // h_s[j] = sum_rb part[(s*hcap + j)*nrb + rb]
+template <typename TV>
 __global__ void __launch_bounds__(kThreads)
-k_lan_reduce_h(LanArgs L, int k, int pass, int nrb) {
+k_lan_reduce_h(LanArgs<TV> L, int k, int pass, int nrb) {
   const int s = blockIdx.y;
   if (!L.active[s]) return;
   if (pass && !L.pass2[s]) return;
@@ -645,85 +735,101 @@ k_lan_reduce_h(LanArgs L, int k, int pass, int nrb) {
 
 // znew -= V(:,0:k) h, column by column in the order of DGEMV 'N'
 // (y_i += (-h_j) V_ij for j = 0..k), with the block partial of |znew|^2.
-// Each thread owns two row pairs (16-byte loads) and walks the columns
-// eight at a time so that 16 loads are in flight per thread.
-constexpr int kGnPairs = 2;                          // row pairs per thread
-constexpr int kGnRows = kThreads * 2 * kGnPairs;     // rows per block (1024)
-constexpr int kGnUnroll = 8;
+// Each thread owns kGnPairs 16-byte row groups and walks the columns
+// kGnUnroll at a time so that 16 loads are in flight per thread.  The
+// running sums are FP64; an FP32 basis is rounded once, when znew is stored.
+template <typename TV>
+constexpr int gn_rows() {  // rows per block: 1024 for both storage types
+  return kThreads * Row16<TV>::R * Row16<TV>::kGnPairs;
+}
 
-template <bool NT>
+template <bool NT, typename TV>
 __global__ void __launch_bounds__(kThreads)
-k_lan_gemvn(LanArgs L, int k, int pass, int nblk) {
+k_lan_gemvn(LanArgs<TV> L, int k, int pass, int nblk) {
 #pragma clang fp contract(off)
+  using RV = Row16<TV>;
+  using vec = typename RV::vec;
+  constexpr int R = RV::R;
+  constexpr int P = RV::kGnPairs;
+  constexpr int U = RV::kGnUnroll;
   const int s = blockIdx.y;
   if (!L.active[s]) return;
   if (pass && !L.pass2[s]) return;
-  double2 *z = reinterpret_cast<double2 *>(lan_col(L, s, k + 1));
+  vec *z = reinterpret_cast<vec *>(lan_col(L, s, k + 1));
   const double *h = L.h + (size_t)s * L.hcap;
-  const size_t npair = L.ld / 2;
-  size_t pr[kGnPairs];
-  double2 acc[kGnPairs];
+  const size_t ngrp = L.ld / R;
+  size_t pr[P];
+  double acc[P][R];
+  bool ok[P];
 #pragma unroll
-  for (int i = 0; i < kGnPairs; i++) {
-    pr[i] = (size_t)blockIdx.x * (kGnRows / 2) + (size_t)i * kThreads + threadIdx.x;
-    acc[i] = pr[i] < npair ? z[pr[i]] : make_double2(0.0, 0.0);
+  for (int i = 0; i < P; i++) {
+    pr[i] = (size_t)blockIdx.x * (gn_rows<TV>() / R) + (size_t)i * kThreads + threadIdx.x;
+    ok[i] = pr[i] < ngrp;
+    const vec zv = ok[i] ? z[pr[i]] : RV::zero();
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[i][r] = RV::get(zv, r);
   }
-  const bool ok0 = pr[0] < npair, ok1 = pr[1] < npair;
   int j = 0;
-  const size_t cstride = L.ld / 2;
-  for (; j + kGnUnroll <= k + 1; j += kGnUnroll) {
-    double2 v0[kGnUnroll], v1[kGnUnroll];
-    double mh[kGnUnroll];
-    // j is a multiple of kGnUnroll: the group lies in one basis chunk
-    const double2 *base = reinterpret_cast<const double2 *>(lan_col(L, s, j));
+  const size_t cstride = L.ld / R;
+  for (; j + U <= k + 1; j += U) {
+    vec v[U][P];
+    double mh[U];
+    // j is a multiple of U (<= 32): the group lies in one basis chunk
+    const vec *base = reinterpret_cast<const vec *>(lan_col(L, s, j));
 #pragma unroll
-    for (int u = 0; u < kGnUnroll; u++) {
-      const double2 *col = base + u * cstride;
+    for (int u = 0; u < U; u++) {
+      const vec *col = base + u * cstride;
       mh[u] = -h[j + u];
-      v0[u] = ok0 ? ld_basis<NT>(col + pr[0]) : make_double2(0.0, 0.0);
-      v1[u] = ok1 ? ld_basis<NT>(col + pr[1]) : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int i = 0; i < P; i++) v[u][i] = ok[i] ? RV::template ld<NT>(col + pr[i]) : RV::zero();
     }
 #pragma unroll
-    for (int u = 0; u < kGnUnroll; u++) {
-      acc[0].x = acc[0].x + mh[u] * v0[u].x;
-      acc[0].y = acc[0].y + mh[u] * v0[u].y;
-      acc[1].x = acc[1].x + mh[u] * v1[u].x;
-      acc[1].y = acc[1].y + mh[u] * v1[u].y;
-    }
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int i = 0; i < P; i++)
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[i][r] = acc[i][r] + mh[u] * RV::get(v[u][i], r);
   }
   for (; j <= k; j++) {
-    const double2 *col = reinterpret_cast<const double2 *>(lan_col(L, s, j));
+    const vec *col = reinterpret_cast<const vec *>(lan_col(L, s, j));
     const double m = -h[j];
-    if (ok0) {
-      const double2 v = ld_basis<NT>(col + pr[0]);
-      acc[0].x = acc[0].x + m * v.x;
-      acc[0].y = acc[0].y + m * v.y;
-    }
-    if (ok1) {
-      const double2 v = ld_basis<NT>(col + pr[1]);
-      acc[1].x = acc[1].x + m * v.x;
-      acc[1].y = acc[1].y + m * v.y;
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+      if (ok[i]) {
+        const vec v = RV::template ld<NT>(col + pr[i]);
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[i][r] = acc[i][r] + m * RV::get(v, r);
+      }
     }
   }
   double sq = 0.0;
 #pragma unroll
-  for (int i = 0; i < kGnPairs; i++) {
-    if (pr[i] < npair) {
-      z[pr[i]] = acc[i];
-      sq += acc[i].x * acc[i].x + acc[i].y * acc[i].y;
+  for (int i = 0; i < P; i++) {
+    if (ok[i]) {
+      RV::store(z + pr[i], acc[i]);
+      // the norm of the stored vector (FP32: after rounding)
+      double st[R];
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        st[r] = std::is_same<TV, float>::value ? (double)(float)acc[i][r] : acc[i][r];
+      double t = st[0] * st[0];
+#pragma unroll
+      for (int r = 1; r < R; r++) t = t + st[r] * st[r];
+      sq += t;
     }
   }
   block_sum_store(sq, L.part + (size_t)s * nblk + blockIdx.x);
 }
 
-__global__ void k_lan_scale(LanArgs L, int k) {
+template <typename TV>
+__global__ void k_lan_scale(LanArgs<TV> L, int k) {
   const int s = blockIdx.y;
   if (!L.active[s]) return;
-  double *z = lan_col(L, s, k + 1);
+  TV *z = lan_col(L, s, k + 1);
   const double t = 1.0 / L.beta[s];
   for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < (size_t)L.n;
        i += (size_t)gridDim.x * kThreads)
-    z[i] = z[i] * t;
+    z[i] = (TV)((double)z[i] * t);
 }
 
 // ---------------------------------------------------------------------------
@@ -1156,73 +1262,117 @@ void fill_zero_f64(size_t n, double *x, hipStream_t s) {
   hipLaunchKernelGGL(k_zero, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x);
 }
 
-template <typename T>
-void pack_cols(int n, int B, int ncol, const double *const *cols, T *blk,
-               hipStream_t s) {
-  hipLaunchKernelGGL((k_pack<T>), dim3(grid_for((size_t)n * B)),
-                     dim3(kThreads), 0, s, n, B, ncol, cols, blk);
+template <typename T, typename TV>
+void pack_cols(int n, int B, int ncol, const TV *const *cols, T *blk, hipStream_t s) {
+  hipLaunchKernelGGL((k_pack<T, TV>), dim3(grid_for((size_t)n * B)), dim3(kThreads), 0, s,
+                     n, B, ncol, cols, blk);
 }
-template <typename T>
-void unpack_cols(int n, int B, int ncol, const T *blk, double *const *cols,
-                 hipStream_t s) {
-  hipLaunchKernelGGL((k_unpack<T>), dim3(grid_for((size_t)n * B)),
-                     dim3(kThreads), 0, s, n, B, ncol, blk, cols);
+template <typename T, typename TV>
+void unpack_cols(int n, int B, int ncol, const T *blk, TV *const *cols, hipStream_t s) {
+  hipLaunchKernelGGL((k_unpack<T, TV>), dim3(grid_for((size_t)n * B)), dim3(kThreads), 0, s,
+                     n, B, ncol, blk, cols);
 }
-template void pack_cols<double>(int, int, int, const double *const *, double *,
-                                hipStream_t);
-template void pack_cols<float>(int, int, int, const double *const *, float *,
-                               hipStream_t);
-template void unpack_cols<double>(int, int, int, const double *,
-                                  double *const *, hipStream_t);
-template void unpack_cols<float>(int, int, int, const float *, double *const *,
-                                 hipStream_t);
+template void pack_cols<double, double>(int, int, int, const double *const *, double *,
+                                        hipStream_t);
+template void pack_cols<float, double>(int, int, int, const double *const *, float *,
+                                       hipStream_t);
+template void pack_cols<float, float>(int, int, int, const float *const *, float *,
+                                      hipStream_t);
+template void unpack_cols<double, double>(int, int, int, const double *, double *const *,
+                                          hipStream_t);
+template void unpack_cols<float, double>(int, int, int, const float *, double *const *,
+                                         hipStream_t);
+template void unpack_cols<float, float>(int, int, int, const float *, float *const *,
+                                        hipStream_t);
+
+int dot_partials_f32(int n, const float *x, const float *y, double *part, hipStream_t s) {
+  const int nb = lan_vec_blocks(n);
+  hipLaunchKernelGGL(k_dot_part_f32, dim3(nb), dim3(kThreads), 0, s, n, x, y, part);
+  return nb;
+}
+
+void scale_vec_f32(int n, float *x, double hostscale, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_scale_f32, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x, hostscale);
+}
 
 size_t lan_part_doubles(int n, int nslice, int hcap) {
+  // sized for FP64 storage, the larger of the two row-block counts
   const size_t ld = ((size_t)n + 63) / 64 * 64;
-  const size_t nrb = (ld + kGtRows - 1) / kGtRows;
-  const size_t nbn = (ld + kGnRows - 1) / kGnRows;
+  const size_t nrb = (ld + gt_rows<double>() - 1) / gt_rows<double>();
+  const size_t nbn = (ld + gn_rows<double>() - 1) / gn_rows<double>();
   const size_t per = std::max(std::max((size_t)hcap * nrb, nbn),
                               (size_t)lan_vec_blocks(n));
   return (size_t)nslice * per + 64;
 }
 
-void lan_alpha(const LanBatch &L, int k, hipStream_t s) {
+namespace {
+template <typename TV>
+void lan_alpha_t(const LanBatch &L, int k, hipStream_t s) {
   const int nb = lan_vec_blocks(L.n);
-  LanArgs a = make_args(L);
-  hipLaunchKernelGGL(k_lan_alpha, dim3(nb, L.nslice), dim3(kThreads), 0, s, a,
-                     k);
-  hipLaunchKernelGGL(k_lan_reduce, dim3(L.nslice), dim3(64), 0, s, a, nb, 0);
+  LanArgs<TV> a = make_args<TV>(L);
+  hipLaunchKernelGGL(k_lan_alpha<TV>, dim3(nb, L.nslice), dim3(kThreads), 0, s, a, k);
+  hipLaunchKernelGGL(k_lan_reduce<TV>, dim3(L.nslice), dim3(64), 0, s, a, nb, 0);
+}
+
+template <typename TV>
+void lan_subtract_alpha_t(const LanBatch &L, int k, hipStream_t s) {
+  const int nb = lan_vec_blocks(L.n);
+  LanArgs<TV> a = make_args<TV>(L);
+  hipLaunchKernelGGL(k_lan_sub_alpha<TV>, dim3(nb, L.nslice), dim3(kThreads), 0, s, a, k);
+  hipLaunchKernelGGL(k_lan_reduce<TV>, dim3(L.nslice), dim3(64), 0, s, a, nb, 1);
+}
+
+template <typename TV>
+void lan_cgs_pass_t(const LanBatch &L, int k, int pass, hipStream_t s) {
+  LanArgs<TV> a = make_args<TV>(L);
+  const int nrb = (int)((L.ld + gt_rows<TV>() - 1) / gt_rows<TV>());
+  const int ncb = (k + 1 + kTcols - 1) / kTcols;
+  // nontemporal basis loads: measured 6.5 vs 5.9 TB/s at k = 4096 (C2)
+  hipLaunchKernelGGL((k_lan_gemvt<true, TV>), dim3(nrb, ncb, L.nslice), dim3(kThreads), 0,
+                     s, a, k, pass, nrb);
+  hipLaunchKernelGGL(k_lan_reduce_h<TV>, dim3(grid_for(k + 1), L.nslice), dim3(kThreads), 0,
+                     s, a, k, pass, nrb);
+  const int nbn = (int)((L.ld + gn_rows<TV>() - 1) / gn_rows<TV>());
+  hipLaunchKernelGGL((k_lan_gemvn<true, TV>), dim3(nbn, L.nslice), dim3(kThreads), 0, s, a,
+                     k, pass, nbn);
+  hipLaunchKernelGGL(k_lan_reduce<TV>, dim3(L.nslice), dim3(64), 0, s, a, nbn, pass ? 3 : 2);
+}
+
+template <typename TV>
+void lan_scale_new_t(const LanBatch &L, int k, hipStream_t s) {
+  LanArgs<TV> a = make_args<TV>(L);
+  const int nb = lan_vec_blocks(L.n);
+  hipLaunchKernelGGL(k_lan_scale<TV>, dim3(nb, L.nslice), dim3(kThreads), 0, s, a, k);
+}
+}  // namespace
+
+void lan_alpha(const LanBatch &L, int k, hipStream_t s) {
+  if (L.vbytes == 4)
+    lan_alpha_t<float>(L, k, s);
+  else
+    lan_alpha_t<double>(L, k, s);
 }
 
 void lan_subtract_alpha(const LanBatch &L, int k, hipStream_t s) {
-  const int nb = lan_vec_blocks(L.n);
-  LanArgs a = make_args(L);
-  hipLaunchKernelGGL(k_lan_sub_alpha, dim3(nb, L.nslice), dim3(kThreads), 0, s,
-                     a, k);
-  hipLaunchKernelGGL(k_lan_reduce, dim3(L.nslice), dim3(64), 0, s, a, nb, 1);
+  if (L.vbytes == 4)
+    lan_subtract_alpha_t<float>(L, k, s);
+  else
+    lan_subtract_alpha_t<double>(L, k, s);
 }
 
 void lan_cgs_pass(const LanBatch &L, int k, int pass, hipStream_t s) {
-  LanArgs a = make_args(L);
-  const int nrb = (int)((L.ld + kGtRows - 1) / kGtRows);
-  const int ncb = (k + 1 + kTcols - 1) / kTcols;
-  // nontemporal basis loads: measured 6.5 vs 5.9 TB/s at k = 4096 (C2)
-  hipLaunchKernelGGL(k_lan_gemvt<true>, dim3(nrb, ncb, L.nslice), dim3(kThreads), 0, s,
-                     a, k, pass, nrb);
-  hipLaunchKernelGGL(k_lan_reduce_h, dim3(grid_for(k + 1), L.nslice),
-                     dim3(kThreads), 0, s, a, k, pass, nrb);
-  const int nbn = (int)((L.ld + kGnRows - 1) / kGnRows);
-  hipLaunchKernelGGL(k_lan_gemvn<true>, dim3(nbn, L.nslice), dim3(kThreads), 0, s, a, k,
-                     pass, nbn);
-  hipLaunchKernelGGL(k_lan_reduce, dim3(L.nslice), dim3(64), 0, s, a, nbn,
-                     pass ? 3 : 2);
+  if (L.vbytes == 4)
+    lan_cgs_pass_t<float>(L, k, pass, s);
+  else
+    lan_cgs_pass_t<double>(L, k, pass, s);
 }
 
 void lan_scale_new(const LanBatch &L, int k, hipStream_t s) {
-  LanArgs a = make_args(L);
-  const int nb = lan_vec_blocks(L.n);
-  hipLaunchKernelGGL(k_lan_scale, dim3(nb, L.nslice), dim3(kThreads), 0, s, a,
-                     k);
+  if (L.vbytes == 4)
+    lan_scale_new_t<float>(L, k, s);
+  else
+    lan_scale_new_t<double>(L, k, s);
 }
 
 void tridiag_trace_above(int m, const double *d, const double *e, double thr,

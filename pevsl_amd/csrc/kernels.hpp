@@ -93,28 +93,35 @@ void copy_f64_to_f32(size_t n, const double *x, float *y, hipStream_t s);
 void copy_f32_to_f64(size_t n, const float *x, double *y, hipStream_t s);
 void fill_zero_f64(size_t n, double *x, hipStream_t s);
 
-// Pack/unpack between one column-major FP64 vector per slice and an
+// Pack/unpack between one column-major vector of type TV per slice and an
 // interleaved n x B block of type T.  cols[c] points to the n-vector of
 // column c (device array of pointers), c < ncol.
-template <typename T>
-void pack_cols(int n, int B, int ncol, const double *const *cols, T *blk,
+template <typename T, typename TV = double>
+void pack_cols(int n, int B, int ncol, const TV *const *cols, T *blk,
                hipStream_t s);
-template <typename T>
-void unpack_cols(int n, int B, int ncol, const T *blk, double *const *cols,
+template <typename T, typename TV = double>
+void unpack_cols(int n, int B, int ncol, const T *blk, TV *const *cols,
                  hipStream_t s);
+// FP32 vector helpers (FP64 accumulation)
+int dot_partials_f32(int n, const float *x, const float *y, double *part,
+                     hipStream_t s);
+void scale_vec_f32(int n, float *x, double hostscale, hipStream_t s);
 
 // ---- Lanczos on a batch of slices ----------------------------------------
 // Slices are independent; slice s owns V_s (chunked columns, leading dim ld
 // a multiple of 64, chunks of 2^chunk_shift >= 32 columns whose rows
 // [n, ld) are zero) and per-step scalars.  Column j of slice s is
 //   vtab[s*maxchunks + (j >> chunk_shift)] + (size_t)(j & chunk_mask)*ld
+// elements of vbytes bytes: FP64 (8) or FP32 (4) storage.  All arithmetic
+// (dots, updates, norms) is FP64 either way.
 struct LanBatch {
   int n = 0;
   size_t ld = 0;
   int nslice = 0;
   int chunk_shift = 9;
   int maxchunks = 0;
-  double **vtab = nullptr;  // device [nslice*maxchunks]
+  int vbytes = 8;           // basis element size: 8 (double) or 4 (float)
+  void **vtab = nullptr;    // device [nslice*maxchunks]
   // device scalars per slice
   double *alpha = nullptr;  // [nslice]
   double *beta = nullptr;   // [nslice] (beta of the previous step)

@@ -31,6 +31,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include <rocsolver/rocsolver.h>
 
@@ -48,7 +49,7 @@ struct Slice {
   double aa = 0, bb = 0, bar = 0, gam = 0;
   int maxit = 0;
   const pevsl_Polparams *pol = nullptr;
-  std::vector<double *> chunks;
+  std::vector<void *> chunks;  // basis chunks (FP64 or FP32 elements)
   std::vector<double> dT, eT;
   double wn = 0.0;
   int nwn = 0;
@@ -70,6 +71,36 @@ size_t filter_launch_bytes(const DevCsr &A, int elem, int ncol, bool first) {
 }
 
 }  // namespace
+
+// FP32 filter y = p((A - cc)/dd) v on FP32 columns; w holds 3n floats.
+static void cheb_filter_f32(Ctx &c, const pevsl_Polparams *pol, const float *x32, float *y32,
+                            float *w) {
+  const DevCsr &A = *c.A;
+  const int n = A.n, m = pol->deg;
+  const double *mu = pol->mu;
+  const double cc = pol->cc, dd = pol->dd;
+  const hipStream_t s = c.stream;
+  const int lanes = c.lanes ? c.lanes : auto_lanes((double)A.nnz / std::max(1, n));
+  const bool stream = c.use_stream != 0;
+  float *b[3] = {w, w + (size_t)n, w + 2 * (size_t)n};
+  const float *vk = x32, *vkm1 = nullptr;
+  int nb = 0;
+  for (int k = 1; k <= m; k++) {
+    float *vkp1 = b[nb];
+    nb = (nb + 1) % 3;
+    if (stream)
+      cheb_step_stream<float>(A, vk, vkm1, vkp1, y32, cc, k == 1 ? 1.0 / dd : 2.0 / dd, mu[0],
+                              mu[k], k == 1, s);
+    else
+      cheb_step_vec<float>(A, vk, vkm1, vkp1, y32, cc, k == 1 ? 1.0 / dd : 2.0 / dd, mu[0],
+                           mu[k], k == 1, lanes, s);
+    vkm1 = vk;
+    vk = vkp1;
+    c.kstats.filter_bytes += (double)filter_launch_bytes(A, 4, 1, k == 1);
+  }
+  c.kstats.filter_steps += m;
+  c.kstats.filter_launches += m;
+}
 
 // y = p((A - cc)/dd) v for one FP64 column in HBM (SRC/chebpol.c:509-574).
 void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
@@ -126,28 +157,19 @@ void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
   } else {
     float *w = reinterpret_cast<float *>(c.scratch64.ensure((size_t)5 * n / 2 + 8));
     float *x32 = w, *y32 = w + n;
-    float *b[3] = {w + 2 * (size_t)n, w + 3 * (size_t)n, w + 4 * (size_t)n};
     copy_f64_to_f32((size_t)n, dv, x32, s);
-    const float *vk = x32, *vkm1 = nullptr;
-    int nb = 0;
-    for (int k = 1; k <= m; k++) {
-      float *vkp1 = b[nb];
-      nb = (nb + 1) % 3;
-      if (stream)
-        cheb_step_stream<float>(A, vk, vkm1, vkp1, y32, cc, k == 1 ? 1.0 / dd : 2.0 / dd,
-                                mu[0], mu[k], k == 1, s);
-      else
-        cheb_step_vec<float>(A, vk, vkm1, vkp1, y32, cc,
-                             k == 1 ? 1.0 / dd : 2.0 / dd, mu[0], mu[k], k == 1,
-                             lanes, s);
-      vkm1 = vk;
-      vk = vkp1;
-      c.kstats.filter_bytes += (double)filter_launch_bytes(A, 4, 1, k == 1);
-    }
+    cheb_filter_f32(c, pol, x32, y32, w + 2 * (size_t)n);
     copy_f32_to_f64((size_t)n, y32, dy, s);
+    return;
   }
   c.kstats.filter_steps += m;
   c.kstats.filter_launches += m;
+}
+
+void cheb_filter_col32(Ctx &c, const pevsl_Polparams *pol, const float *dv, float *dy) {
+  const int n = c.A->n;
+  float *w = reinterpret_cast<float *>(c.scratch64.ensure((size_t)3 * n / 2 + 8));
+  cheb_filter_f32(c, pol, dv, dy, w);
 }
 
 namespace {
@@ -162,10 +184,18 @@ class BatchSolver {
                pevsl_Parvec *vinit);
 
  private:
-  double *col(int s, int j) {
+  void *col(int s, int j) {
     const int sh = L_.chunk_shift;
-    return S_[(size_t)s].chunks[(size_t)(j >> sh)] + (size_t)(j & ((1 << sh) - 1)) * L_.ld;
+    return static_cast<char *>(S_[(size_t)s].chunks[(size_t)(j >> sh)]) +
+           (size_t)(j & ((1 << sh) - 1)) * L_.ld * vb_;
   }
+  double *cold(int s, int j) { return static_cast<double *>(col(s, j)); }
+  float *colf(int s, int j) { return static_cast<float *>(col(s, j)); }
+  // FP64 view of basis chunk ci of slice s (kc columns): the chunk itself, or
+  // its widened copy in conv_ when the basis is stored in FP32
+  const double *chunk64(int s, int ci, int kc);
+  // z = p(A) x between basis columns / an FP64 input, in the basis' type
+  void filter_into(int s, const double *x64, int kin, int kout);
   void ensure_column(int s, int j);
   void sync_vtab();
   void set_active(const std::vector<int> &act);
@@ -183,31 +213,58 @@ class BatchSolver {
   pevsl_Data *p_;
   int ns_;
   int n_ = 0;
+  bool b32_ = false;  // FP32 basis storage
+  size_t vb_ = 8;     // basis element bytes
   std::vector<Slice> S_;
   LanBatch L_;
-  DBuf<double *> vtab_;
-  std::vector<double *> vtab_h_;
+  DBuf<void *> vtab_;
+  std::vector<void *> vtab_h_;
+  DBuf<double> conv_;   // FP64 copy of one FP32 basis chunk (Ritz stage)
+  DBuf<float> f32tmp_;  // FP32 staging of an FP64 input vector
   DBuf<double> scal_;  // alpha, beta, nrm2a, nrm2b
   DBuf<int> flags_;    // pass2, active
   DBuf<double> h_, part_;
   DBuf<double> blk_;   // filter blocks
   DBuf<double> mus_;
-  DBuf<double *> colptr_;
+  DBuf<void *> colptr_;
   DBuf<double> tri_;   // tridiagonal work
   double *pinned_ = nullptr;
   int *pinned_i_ = nullptr;
   std::vector<int> mus_key_;
   std::vector<double> mus_h_, betas_h_;
-  std::vector<double *> cp_h_;
+  std::vector<void *> cp_h_;
 };
+
+const double *BatchSolver::chunk64(int s, int ci, int kc) {
+  void *p = S_[(size_t)s].chunks[(size_t)ci];
+  if (!b32_) return static_cast<const double *>(p);
+  double *d = conv_.ensure(L_.ld * ((size_t)1 << L_.chunk_shift));
+  copy_f32_to_f64(L_.ld * (size_t)kc, static_cast<const float *>(p), d, c_.stream);
+  return d;
+}
+
+void BatchSolver::filter_into(int s, const double *x64, int kin, int kout) {
+  const pevsl_Polparams *pol = S_[(size_t)s].pol;
+  if (!b32_) {
+    cheb_filter_col(c_, pol, x64 ? x64 : cold(s, kin), cold(s, kout));
+    return;
+  }
+  const float *x = kin >= 0 ? colf(s, kin) : nullptr;
+  if (x64) {
+    f32tmp_.ensure((size_t)n_);
+    copy_f64_to_f32((size_t)n_, x64, f32tmp_.p, c_.stream);
+    x = f32tmp_.p;
+  }
+  cheb_filter_col32(c_, pol, x, colf(s, kout));
+}
 
 void BatchSolver::ensure_column(int s, int j) {
   Slice &sl = S_[(size_t)s];
   const size_t need = (size_t)(j >> L_.chunk_shift) + 1;
   bool grew = false;
   while (sl.chunks.size() < need) {
-    double *p = nullptr;
-    const size_t bytes = sizeof(double) * L_.ld * ((size_t)1 << L_.chunk_shift);
+    void *p = nullptr;
+    const size_t bytes = vb_ * L_.ld * ((size_t)1 << L_.chunk_shift);
     PV_HIP(hipMalloc(&p, bytes));
     // rows [n, ld) of every column stay zero: the reorthogonalization
     // kernels stream whole row pairs up to ld
@@ -220,7 +277,7 @@ void BatchSolver::ensure_column(int s, int j) {
 }
 
 void BatchSolver::sync_vtab() {
-  PV_HIP(hipMemcpyAsync(vtab_.p, vtab_h_.data(), sizeof(double *) * vtab_h_.size(),
+  PV_HIP(hipMemcpyAsync(vtab_.p, vtab_h_.data(), sizeof(void *) * vtab_h_.size(),
                         hipMemcpyHostToDevice, c_.stream));
   PV_HIP(hipStreamSynchronize(c_.stream));
 }
@@ -248,11 +305,11 @@ void BatchSolver::free_basis(int s) {
 void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
   const DevCsr &A = *c_.A;
   const int g = (int)grp.size();
-  int B = c_.block_width;
-  if (B == 0 || B < g) {
-    B = 1;
-    while (B < g) B *= 2;
-  }
+  // the narrowest kernel width that holds the group (groups never exceed
+  // the tuned width): a width-8 block holding 4 columns would stream twice
+  // the vector bytes it needs
+  int B = 1;
+  while (B < g) B *= 2;
   const bool f32 = c_.filter_prec == PEVSL_PREC_FP32;
   const size_t elem = f32 ? 4 : 8;
   const size_t blk_elems = (size_t)n_ * B;
@@ -284,14 +341,14 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
   }
   // column pointers: inputs z_k, outputs z_{k+1}
   PV_HIP(hipStreamSynchronize(c_.stream));
-  std::vector<double *> &cp = cp_h_;
+  std::vector<void *> &cp = cp_h_;
   cp.assign((size_t)2 * B, nullptr);
   for (int ci = 0; ci < g; ci++) {
     cp[(size_t)ci] = col(grp[(size_t)ci], k);
     cp[(size_t)B + ci] = col(grp[(size_t)ci], k + 1);
   }
   colptr_.ensure(cp.size());
-  PV_HIP(hipMemcpyAsync(colptr_.p, cp.data(), sizeof(double *) * cp.size(),
+  PV_HIP(hipMemcpyAsync(colptr_.p, cp.data(), sizeof(void *) * cp.size(),
                         hipMemcpyHostToDevice, c_.stream));
   const double cc = S_[(size_t)grp[0]].pol->cc, dd = S_[(size_t)grp[0]].pol->dd;
   std::vector<int> degs;
@@ -302,7 +359,17 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
     using T = std::remove_pointer_t<decltype(tag)>;
     T *x = (T *)xb[0], *y = (T *)xb[1];
     T *b[3] = {(T *)xb[2], (T *)xb[3], nullptr};
-    pack_cols<T>(n_, B, g, colptr_.p, x, c_.stream);
+    if constexpr (std::is_same<T, float>::value) {
+      if (b32_)
+        pack_cols<float, float>(n_, B, g, reinterpret_cast<const float *const *>(colptr_.p), x,
+                                c_.stream);
+      else
+        pack_cols<float, double>(n_, B, g, reinterpret_cast<const double *const *>(colptr_.p),
+                                 x, c_.stream);
+    } else {
+      pack_cols<T, double>(n_, B, g, reinterpret_cast<const double *const *>(colptr_.p), x,
+                           c_.stream);
+    }
     // third work buffer: reuse x once it is no longer read (after step 2)
     const T *vk = x, *vkm1 = nullptr;
     T *bufs[3] = {b[0], b[1], x};
@@ -329,7 +396,17 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
       vkm1 = vk;
       vk = vkp1;
     }
-    unpack_cols<T>(n_, B, g, y, colptr_.p + B, c_.stream);
+    if constexpr (std::is_same<T, float>::value) {
+      if (b32_)
+        unpack_cols<float, float>(n_, B, g, y, reinterpret_cast<float *const *>(colptr_.p + B),
+                                  c_.stream);
+      else
+        unpack_cols<float, double>(n_, B, g, y,
+                                   reinterpret_cast<double *const *>(colptr_.p + B), c_.stream);
+    } else {
+      unpack_cols<T, double>(n_, B, g, y, reinterpret_cast<double *const *>(colptr_.p + B),
+                             c_.stream);
+    }
   };
   if (f32)
     run((float *)nullptr);
@@ -342,7 +419,7 @@ void BatchSolver::filter_step(const std::vector<int> &act, int k) {
   const bool single = act.size() == 1 || !c_.A || c_.block_width == 1;
   if (single) {
     for (int s : act) {
-      cheb_filter_col(c_, S_[(size_t)s].pol, col(s, k), col(s, k + 1));
+      filter_into(s, nullptr, k, k + 1);
       S_[(size_t)s].nmatvec += S_[(size_t)s].pol->deg;
     }
     return;
@@ -352,12 +429,15 @@ void BatchSolver::filter_step(const std::vector<int> &act, int k) {
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return S_[(size_t)a].pol->deg > S_[(size_t)b].pol->deg;
   });
-  const int gmax = c_.block_width ? c_.block_width : 64;
+  // group width: the tuned one, else 8 (FP32) / 4 (FP64), which pEVSL_Autotune
+  // picks on the 7-point stencil (wider row gathers cost more per column)
+  const int gmax = c_.block_width ? c_.block_width
+                                  : (c_.filter_prec == PEVSL_PREC_FP32 ? 8 : 4);
   for (size_t i = 0; i < order.size(); i += (size_t)gmax) {
     std::vector<int> grp(order.begin() + (long)i,
                          order.begin() + (long)std::min(order.size(), i + (size_t)gmax));
     if (grp.size() == 1) {
-      cheb_filter_col(c_, S_[(size_t)grp[0]].pol, col(grp[0], k), col(grp[0], k + 1));
+      filter_into(grp[0], nullptr, k, k + 1);
     } else {
       filter_block(grp, k);
     }
@@ -370,7 +450,7 @@ void BatchSolver::lanczos_update(const std::vector<int> &act, int k) {
   set_active(act);
   lan_alpha(L_, k, c_.stream);
   lan_subtract_alpha(L_, k, c_.stream);
-  const double vecb = 8.0 * n_;
+  const double vecb = (double)vb_ * n_;
   c_.kstats.reorth_bytes += act.size() * (4 * vecb + 3 * vecb);
 }
 
@@ -379,7 +459,7 @@ void BatchSolver::lanczos_update(const std::vector<int> &act, int k) {
 void BatchSolver::cgs(const std::vector<int> &act, int k) {
   lan_cgs_pass(L_, k, 0, c_.stream);
   lan_cgs_pass(L_, k, 1, c_.stream);
-  const double vecb = 8.0 * n_;
+  const double vecb = (double)vb_ * n_;
   const double pass = 2.0 * (k + 1) * vecb + 3 * vecb;
   c_.kstats.reorth_bytes += act.size() * pass;
   c_.kstats.cgs_bytes += act.size() * pass;
@@ -393,7 +473,7 @@ void BatchSolver::breakdown(int s, int k) {
   sl.nbreak++;
   double *vr = c_.cb_x.ensure((size_t)n_);
   rand_vectors(c_, n_, 1, vr, (size_t)n_);
-  cheb_filter_col(c_, sl.pol, vr, col(s, k + 1));
+  filter_into(s, vr, -1, k + 1);
   sl.nmatvec += sl.pol->deg;
   std::vector<int> one{s};
   set_active(one);
@@ -431,6 +511,7 @@ void BatchSolver::convergence_test(int s, int k) {
                         c_.stream));
   PV_HIP(hipStreamSynchronize(c_.stream));
   p_->stats->t_eig += wtime() - t0;
+  c_.kstats.t_eig += wtime() - t0;
   sl.kdim = kdim;
   sl.tr1 = pinned_[0];
   sl.nconv = (int)pinned_[1];
@@ -467,6 +548,7 @@ void BatchSolver::ritz(int s, double tol) {
   PV_HIP(hipStreamSynchronize(st));
   if (hinfo) fprintf(stderr, "pevsl_amd: STEDC info %d\n", hinfo);
   p_->stats->t_eig += wtime() - t0;
+  c_.kstats.t_eig += wtime() - t0;
   int i0 = kdim;
   while (i0 > 0 && ev[(size_t)i0 - 1] >= sl.bar) i0--;
   const int nsel = kdim - i0;
@@ -489,7 +571,7 @@ void BatchSolver::ritz(int s, double tol) {
     for (int j0 = 0, ci = 0; j0 < kdim; j0 += csz, ci++) {
       const int kc = std::min(csz, kdim - j0);
       PV_BLAS(rocblas_dgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n_, nb,
-                            kc, &one, sl.chunks[(size_t)ci], (int)ld,
+                            kc, &one, chunk64(s, ci, kc), (int)ld,
                             C.p + (size_t)(i0 + cs) * kdim + j0, kdim,
                             ci == 0 ? &zero : &one, U.p, (int)ld));
     }
@@ -544,6 +626,7 @@ void BatchSolver::ritz(int s, double tol) {
     PV_HIP(hipStreamSynchronize(st));
   }
   p_->stats->t_ritz += wtime() - t0;
+  c_.kstats.t_ritz += wtime() - t0;
 }
 
 
@@ -579,6 +662,7 @@ void BatchSolver::ritz_rr(int s, double tol) {
   while (i0 > 0 && ev[(size_t)i0 - 1] >= sl.bar) i0--;
   const int nsel = kdim - i0;
   p_->stats->t_eig += wtime() - t0;
+  c_.kstats.t_eig += wtime() - t0;
   if (nsel == 0) return;
   const size_t ld = L_.ld;
   const int csz = 1 << L_.chunk_shift;
@@ -589,11 +673,12 @@ void BatchSolver::ritz_rr(int s, double tol) {
   for (int j0 = 0, ci = 0; j0 < kdim; j0 += csz, ci++) {
     const int kc = std::min(csz, kdim - j0);
     PV_BLAS(rocblas_dgemm(c_.blas, rocblas_operation_none, rocblas_operation_none, n_, nsel, kc,
-                          &one, sl.chunks[(size_t)ci], (int)ld, C.p + (size_t)i0 * kdim + j0,
+                          &one, chunk64(s, ci, kc), (int)ld, C.p + (size_t)i0 * kdim + j0,
                           kdim, ci == 0 ? &zero : &one, U.p, (int)ld));
   }
   C.release();
   free_basis(s);  // the basis is no longer needed
+  conv_.release();
   const int lanes = c_.lanes ? c_.lanes : auto_lanes((double)c_.A->nnz / std::max(1, n_));
   spmv_cols_f64(*c_.A, U.p, W.p, nsel, ld, lanes, st);
   sl.nmatvec += nsel;
@@ -667,6 +752,7 @@ void BatchSolver::ritz_rr(int s, double tol) {
   }
   PV_HIP(hipStreamSynchronize(st));
   p_->stats->t_ritz += wtime() - t0;
+  c_.kstats.t_ritz += wtime() - t0;
 }
 
 int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit,
@@ -699,6 +785,10 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
     sl.eT.assign((size_t)sl.maxit, 0.0);
     kmax = std::max(kmax, sl.maxit);
   }
+  // FP32 basis storage: only with the FP32 filter on the device matrix
+  b32_ = c_.basis_prec == PEVSL_PREC_FP32 && c_.filter_prec == PEVSL_PREC_FP32 && c_.A;
+  vb_ = b32_ ? 4 : 8;
+  L_.vbytes = (int)vb_;
   // batch workspace
   int shift = 0;
   while ((1 << shift) < std::max(64, c_.chunk_cols)) shift++;
@@ -736,7 +826,7 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
   double t_filter = 0, t_reorth = 0, t_cgs = 0;
   for (int s : act) {
     ensure_column(s, 0);
-    cheb_filter_col(c_, S_[(size_t)s].pol, dvi, col(s, 0));
+    filter_into(s, dvi, -1, 0);
     S_[(size_t)s].nmatvec += S_[(size_t)s].pol->deg;
   }
   if (!act.empty()) {
@@ -746,11 +836,15 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
     DBuf<double> pt;
     pt.ensure((size_t)dot_part_blocks(n_) + 8);
     for (int s : act) {
-      const int nb = dot_partials(n_, col(s, 0), col(s, 0), pt.p, st);
+      const int nb = b32_ ? dot_partials_f32(n_, colf(s, 0), colf(s, 0), pt.p, st)
+                          : dot_partials(n_, cold(s, 0), cold(s, 0), pt.p, st);
       reduce_partials(pt.p, nb, 1, pt.p + nb, st);
       PV_HIP(hipMemcpyAsync(pinned_, pt.p + nb, sizeof(double), hipMemcpyDeviceToHost, st));
       PV_HIP(hipStreamSynchronize(st));
-      scale_vec(n_, col(s, 0), nullptr, 1.0 / sqrt(pinned_[0]), st);
+      if (b32_)
+        scale_vec_f32(n_, colf(s, 0), 1.0 / sqrt(pinned_[0]), st);
+      else
+        scale_vec(n_, cold(s, 0), nullptr, 1.0 / sqrt(pinned_[0]), st);
     }
   }
   for (int k = 0; !act.empty(); k++) {
@@ -779,7 +873,7 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
     t_cgs += ev_c.seconds();
     for (int s : act) {
       if (pinned_i_[s]) {  // second DGKS pass ran
-        const double pass = 2.0 * (k + 1) * 8.0 * n_ + 3 * 8.0 * n_;
+        const double pass = 2.0 * (k + 1) * (double)vb_ * n_ + 3.0 * (double)vb_ * n_;
         c_.kstats.reorth_bytes += pass;
         c_.kstats.cgs_bytes += pass;
         c_.kstats.cgs_passes++;

@@ -153,13 +153,26 @@ int pEVSL_BenchReorth(pevsl_Data *p, int k, int reps, double *sec, double *bytes
   while ((1 << shift) < k + 1 || shift < 5) shift++;
   L.chunk_shift = shift;
   L.maxchunks = 1;
+  // basis storage as the instance's Lanczos runs use it
+  const bool b32 = c.basis_prec == PEVSL_PREC_FP32;
+  L.vbytes = b32 ? 4 : 8;
+  const size_t cnt = L.ld * ((size_t)1 << shift);
   DBuf<double> V, sc, h, part;
-  DBuf<double *> vt;
+  DBuf<float> V32;
+  DBuf<void *> vt;
   DBuf<int> fl;
-  V.ensure(L.ld * ((size_t)1 << shift));
-  rand_fill(L.ld * ((size_t)1 << shift), V.p, 7, 0, 0, c.stream);
+  V.ensure(cnt);
+  rand_fill(cnt, V.p, 7, 0, 0, c.stream);
+  void *vp = V.p;
+  if (b32) {
+    V32.ensure(cnt);
+    copy_f64_to_f32(cnt, V.p, V32.p, c.stream);
+    PV_HIP(hipStreamSynchronize(c.stream));
+    V.release();
+    vp = V32.p;
+  }
   L.vtab = vt.ensure(1);
-  PV_HIP(hipMemcpy(L.vtab, &V.p, sizeof(double *), hipMemcpyHostToDevice));
+  PV_HIP(hipMemcpy(L.vtab, &vp, sizeof(void *), hipMemcpyHostToDevice));
   double *s = sc.ensure(4);
   L.alpha = s;
   L.beta = s + 1;
@@ -182,7 +195,7 @@ int pEVSL_BenchReorth(pevsl_Data *p, int k, int reps, double *sec, double *bytes
   for (int r = 0; r < reps; r++) lan_cgs_pass(L, kk, 0, c.stream);
   ev.stop(c.stream);
   *sec = ev.seconds() / reps;
-  *bytes = 2.0 * 8.0 * (double)n * k + 4.0 * 8.0 * n;
+  *bytes = 2.0 * L.vbytes * (double)n * k + 4.0 * L.vbytes * n;
   return 0;
 }
 

@@ -106,7 +106,7 @@ class Instance:
     def __init__(self, lib=None, matrix=None, prec=capi.PEVSL_PREC_FP64,
                  rand_mode=capi.PEVSL_RAND_LIBC, seed=0, keep_vectors=False,
                  lanes=0, block_width=0, ritz_refine=-1, mixed_tol=None,
-                 block_rowkernel=-1):
+                 block_rowkernel=-1, basis_prec=capi.PEVSL_PREC_FP64):
         self.lib = lib or capi.load()
         self.p = capi.DataP()
         rc = self.lib.pEVSL_Start(capi.MPI_COMM_WORLD, C.byref(self.p))
@@ -120,6 +120,7 @@ class Instance:
         self.lib.pEVSL_SetSpmvTuning(self.p, lanes, block_width)
         self.lib.pEVSL_SetRitzRefine(self.p, ritz_refine)
         self.lib.pEVSL_SetBlockKernel(self.p, block_rowkernel)
+        self.lib.pEVSL_SetBasisPrecision(self.p, basis_prec)
         if mixed_tol is not None:
             self.lib.pEVSL_SetMixedTol(self.p, mixed_tol)
 
@@ -268,7 +269,8 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
                     keep_vectors=False, rand_mode=capi.PEVSL_RAND_LIBC, seed=0,
                     sli_override=None, block_width=0, log=None, verbose=False,
                     maxit_cap=None, lanes=0, ritz_refine=-1, mixed_tol=None,
-                    block_rowkernel=-1, mlp=None, judge=True, judge_tol=0.15, wave=None):
+                    block_rowkernel=-1, mlp=None, judge=True, judge_tol=0.15, wave=None,
+                    basis_prec=capi.PEVSL_PREC_FP64):
     """End-to-end run of the TESTS/Lap/LapPLanN.c flow on the GPU.  Returns a
     dict with the DOS, slices, per-slice filters, eigenvalues and timings.
 
@@ -290,7 +292,8 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
     tol = 1e-8
     inst = Instance(lib, M, prec=prec, rand_mode=rand_mode, seed=seed,
                     block_width=block_width, lanes=lanes, ritz_refine=ritz_refine,
-                    mixed_tol=mixed_tol, block_rowkernel=block_rowkernel)
+                    mixed_tol=mixed_tol, block_rowkernel=block_rowkernel,
+                    basis_prec=basis_prec)
     out = {"n": n, "nnz": M.nnz, "t_setup": time.time() - t0}
     try:
         xintv = np.array([a, b, lmin, lmax])

@@ -335,6 +335,42 @@ def test_mixed_precision_c1(lib, batch):
         assert np.all(s["res"] < 1e-5)
 
 
+@pytest.mark.parametrize("batch", [False, True])
+def test_mixed_precision_fp32_basis_c1(lib, batch):
+    banner("C1 mixed precision with FP32 basis storage batch=%s" % batch)
+    # same stated tolerance as the mixed path: residual < 1e-5, eigenvalue
+    # error < 1e-10 against the closed form
+    r = driver.lap_slicing_run(20, 20, 20, 0.6, 1.2, 5, batch=batch, prec=capi.PEVSL_PREC_FP32,
+                               basis_prec=capi.PEVSL_PREC_FP32)
+    for s in r["slices"]:
+        assert s["nev"] == s["nev_exact"]
+        assert s["max_rel_err"] < 1e-10
+        assert np.all(s["res"] < 1e-5)
+    ks = r["kstats"]
+    # FP32 storage: the reorthogonalization streams 4-byte elements
+    assert ks["cgs_bytes"] > 0
+
+
+def test_fp32_basis_reorth_bytes_halved(lib):
+    banner("BenchReorth FP64 vs FP32 basis, 40^3, k = 256")
+    ia, ja, a = driver.laplacian3d(40, 40, 40)
+    M = driver.Matrix(lib, ia, ja, a)
+    out = {}
+    try:
+        for bp in (capi.PEVSL_PREC_FP64, capi.PEVSL_PREC_FP32):
+            inst = driver.Instance(lib, M, prec=capi.PEVSL_PREC_FP32, basis_prec=bp)
+            try:
+                sec, byt = C.c_double(), C.c_double()
+                assert lib.pEVSL_BenchReorth(inst.p, 256, 5, C.byref(sec), C.byref(byt)) == 0
+                out[bp] = (sec.value, byt.value)
+            finally:
+                inst.close()
+    finally:
+        M.free()
+    assert abs(out[capi.PEVSL_PREC_FP32][1] / out[capi.PEVSL_PREC_FP64][1] - 0.5) < 1e-12
+    assert out[capi.PEVSL_PREC_FP32][0] > 0
+
+
 def test_fp64_rayleigh_ritz_refinement(lib):
     banner("C1 FP64 with the Rayleigh-Ritz Ritz stage")
     r = driver.lap_slicing_run(20, 20, 20, 0.6, 1.2, 5, batch=True, ritz_refine=1)
