@@ -93,12 +93,17 @@ int pEVSL_SetKeepVectors(pevsl_Data *pevsl, int keep);
 /* Kernel of the one-column filter step: lanes per row of the vector-CSR
  * kernel (1..64, power of two), PEVSL_SPMV_STREAM for the CSR-stream kernel
  * (row blocks staged in LDS), or 0 = automatic; and the block width of
- * batched solves (1..64, power of two; 0 = number of slices rounded up). */
+ * batched solves (1..64, power of two; 0 = 8 columns per block for an
+ * FP32 filter and 4 for FP64, the widths pEVSL_Autotune picks on the
+ * 7-point stencil). */
 #define PEVSL_SPMV_STREAM 128
 int pEVSL_SetSpmvTuning(pevsl_Data *pevsl, int lanes_per_row, int block_width);
 
 /* Block SpMM kernel of batched solves: 1 = one thread per row with B-wide
- * row gathers (default, B <= 32), 0 = one lane per (row, column). */
+ * row gathers (B <= 32) and the reference's three-term recurrence with an
+ * accumulator (5 vector passes per degree), 2 = the same kernel in Clenshaw
+ * form (4 passes), 0 = one lane per (row, column).  Default (-1): 2 for an
+ * FP32 filter, 1 for FP64. */
 int pEVSL_SetBlockKernel(pevsl_Data *pevsl, int rowkernel);
 
 /* Eigenvalues >= thr of the symmetric tridiagonal matrix (diag[0..n-1],
@@ -157,8 +162,10 @@ typedef struct {
   double bytes_block[6];
   double t_stream;    /* CSR-stream kernel, seconds per step */
   double t_blockrow[6]; /* row-per-thread block kernel, widths 2..64 (0: n/a) */
-  int block_rowkernel;  /* chosen block kernel: 1 row-per-thread, 0 lane per
-                           (row, column) */
+  int block_rowkernel;  /* chosen block kernel: 2 row-per-thread Clenshaw,
+                           1 row-per-thread, 0 lane per (row, column) */
+  double t_blockclen[6];  /* Clenshaw row kernel, widths 2..64 (FP32 only) */
+  double bytes_clen[6];   /* its algorithmic bytes per step */
 } pevsl_TuneResult;
 int pEVSL_Autotune(pevsl_Data *pevsl, int reps, pevsl_TuneResult *res);
 

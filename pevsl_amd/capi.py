@@ -93,7 +93,8 @@ class TuneResult(C.Structure):
                 ("t_lanes", C.c_double * 6), ("bytes_lanes", C.c_double),
                 ("t_block", C.c_double * 6), ("bytes_block", C.c_double * 6),
                 ("t_stream", C.c_double), ("t_blockrow", C.c_double * 6),
-                ("block_rowkernel", C.c_int)]
+                ("block_rowkernel", C.c_int), ("t_blockclen", C.c_double * 6),
+                ("bytes_clen", C.c_double * 6)]
 
 
 class KernelStats(C.Structure):

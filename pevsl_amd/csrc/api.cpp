@@ -270,7 +270,7 @@ int pEVSL_SetSpmvTuning(pevsl_Data *p, int lanes, int bw) {
 }
 
 int pEVSL_SetBlockKernel(pevsl_Data *p, int rowkernel) {
-  if (rowkernel < -1 || rowkernel > 1) return -1;
+  if (rowkernel < -1 || rowkernel > 2) return -1;
   ctx_of(p).blk_rowkernel = rowkernel;
   return 0;
 }

@@ -366,6 +366,80 @@ k_cheb_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
   store_row<T, B>(y + (size_t)r * B, yy);
 }
 
+// One Clenshaw step of the filter on B interleaved columns (row per thread,
+// a/ja staged in LDS as in k_cheb_blkrow):
+//   out = mu_k v + t (A b1 - cc b1) - b2      (b2 may be null: zero)
+// with per-column mu_k = mus[k*B + c].  Run from the top degree down with
+// t = 2/dd it produces the Clenshaw sums b_k of p((A-cc)/dd) v =
+// sum_k mu_k T_k((A-cc)/dd) v; the last step (k = 0, t = 1/dd) gives the
+// filtered vector itself.  Per step it gathers b1 and streams b2, v and out:
+// four vector passes where the three-term form with an accumulator needs
+// five.
+template <typename T, int B>
+__global__ void __launch_bounds__(kThreads)
+k_clen_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
+              const int *__restrict__ ja, const T *__restrict__ a,
+              const T *__restrict__ b1, const T *__restrict__ b2,
+              const T *__restrict__ v, T *__restrict__ out, T cc, T t,
+              const double *__restrict__ mus, int k) {
+#pragma clang fp contract(off)
+  __shared__ T as[kStreamNnz];
+  __shared__ int js[kStreamNnz];
+  const int rb = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = rowblk[rb], r1 = rowblk[rb + 1];
+  const int j0 = ia[r0], nz = ia[r1] - j0;
+  const bool staged = nz <= kStreamNnz;
+  if (staged) {
+    for (int q = threadIdx.x; q < nz; q += kThreads) {
+      as[q] = a[j0 + q];
+      js[q] = ja[j0 + q];
+    }
+    __syncthreads();
+  } else if (threadIdx.x != 0) {
+    return;  // one long row: handled by thread 0 from global memory
+  }
+  const int r = r0 + threadIdx.x;
+  if (r >= r1) return;
+  T acc[B];
+#pragma unroll
+  for (int c = 0; c < B; c++) acc[c] = 0;
+  const int qs = ia[r] - j0, qe = ia[r + 1] - j0;
+  for (int q = qs; q < qe; q++) {
+    const T av = staged ? as[q] : a[j0 + q];
+    const int col = staged ? js[q] : ja[j0 + q];
+    T x[B];
+    load_row<T, B>(b1 + (size_t)col * B, x);
+#pragma unroll
+    for (int c = 0; c < B; c++) acc[c] += av * x[c];
+  }
+  T bb[B], vv[B], w[B], b2r[B];
+  load_row<T, B>(b1 + (size_t)r * B, bb);
+  load_row<T, B>(v + (size_t)r * B, vv);
+  if (b2) {
+    load_row<T, B>(b2 + (size_t)r * B, b2r);
+  } else {
+#pragma unroll
+    for (int c = 0; c < B; c++) b2r[c] = 0;
+  }
+#pragma unroll
+  for (int c = 0; c < B; c++) {
+    T z = acc[c] + (-cc) * bb[c];
+    z = z * t;
+    z = z + (T)mus[(size_t)k * B + c] * vv[c];
+    w[c] = z + (T)(-1) * b2r[c];
+  }
+  store_row<T, B>(out + (size_t)r * B, w);
+}
+
+// out[r*B + c] = mus[k*B + c] * v[r*B + c]  (top Clenshaw term)
+template <typename T>
+__global__ void k_blk_colscale(size_t n, int B, const T *__restrict__ v, T *__restrict__ out,
+                               const double *__restrict__ mus, int k) {
+  const size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (g >= n * (size_t)B) return;
+  out[g] = (T)mus[(size_t)k * B + (int)(g % B)] * v[g];
+}
+
 // KPM step + per-column partial dot with v0 over the block's rows.
 template <typename T, int B>
 __global__ void __launch_bounds__(kThreads)
@@ -1157,6 +1231,35 @@ void cheb_step_blkrow(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp
       break;
   }
 }
+template <typename T>
+void clen_step_blkrow(const DevCsr &A, int B, const T *b1, const T *b2, const T *v, T *out,
+                      double cc, double t, const double *mus, int k, hipStream_t s) {
+  if (A.n == 0 || A.nrowblk == 0) return;
+  if (!b1) {  // top term: no product
+    hipLaunchKernelGGL((k_blk_colscale<T>), dim3(grid_for((size_t)A.n * B)), dim3(kThreads), 0,
+                       s, (size_t)A.n, B, v, out, mus, k);
+    return;
+  }
+  const T *a = std::is_same<T, double>::value ? (const T *)A.a64 : (const T *)A.a32;
+  switch (B) {
+#define CASE(BB)                                                                         \
+  case BB:                                                                               \
+    hipLaunchKernelGGL((k_clen_blkrow<T, BB>), dim3(A.nrowblk), dim3(kThreads), 0, s,    \
+                       A.rowblk, A.ia, A.ja, a, b1, b2, v, out, (T)cc, (T)t, mus, k);    \
+    break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32)
+#undef CASE
+    default:
+      break;
+  }
+}
+template void clen_step_blkrow<float>(const DevCsr &, int, const float *, const float *,
+                                      const float *, float *, double, double, const double *,
+                                      int, hipStream_t);
+template void clen_step_blkrow<double>(const DevCsr &, int, const double *, const double *,
+                                       const double *, double *, double, double,
+                                       const double *, int, hipStream_t);
+
 template void cheb_step_blkrow<double>(const DevCsr &, int, const double *,
                                        const double *, double *, double *, double,
                                        double, const double *, int, int, hipStream_t);

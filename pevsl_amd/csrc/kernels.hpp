@@ -69,6 +69,13 @@ void cheb_step_blkrow(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp
                       T *y, double cc, double t, const double *mus, int k,
                       int nact, hipStream_t s);
 
+// Clenshaw step on B interleaved columns (row per thread, B <= 32):
+//   out = mu_k v + t (A b1 - cc b1) - b2,  mu_k per column = mus[k*B + c];
+// b2 == nullptr reads as zero; b1 == nullptr gives the top term out = mu_k v.
+template <typename T>
+void clen_step_blkrow(const DevCsr &A, int B, const T *b1, const T *b2, const T *v, T *out,
+                      double cc, double t, const double *mus, int k, hipStream_t s);
+
 // KPM step on B interleaved columns:
 //   vkp1 = scal (A vk - ctr vk) - vkm1,   part[blk][c] = sum_rows vkp1*v0
 // returns the number of row blocks (partials per column) written.

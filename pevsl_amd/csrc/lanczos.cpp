@@ -64,6 +64,13 @@ struct Slice {
   std::vector<double> vecs;  // accepted Ritz vectors (host, n x nev)
 };
 
+// Clenshaw step: gathered b1 (counted once), b2 (if any), v, out
+size_t clen_launch_bytes(const DevCsr &A, int elem, int ncol, bool has_b2) {
+  const double vec = (double)A.n * elem;
+  return (size_t)((double)A.nnz * (elem + 4) + 4.0 * (A.n + 1) +
+                  (has_b2 ? 4.0 : 3.0) * vec * ncol);
+}
+
 size_t filter_launch_bytes(const DevCsr &A, int elem, int ncol, bool first) {
   const double vec = (double)A.n * elem;
   return (size_t)((double)A.nnz * (elem + 4) + 4.0 * (A.n + 1) +
@@ -353,8 +360,11 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
   const double cc = S_[(size_t)grp[0]].pol->cc, dd = S_[(size_t)grp[0]].pol->dd;
   std::vector<int> degs;
   for (int s : grp) degs.push_back(S_[(size_t)s].pol->deg);
-  // row-per-thread kernel (default) up to 32 columns
+  // row-per-thread kernel (default) up to 32 columns; its Clenshaw form is
+  // the default on the FP32 path (the FP64 path keeps the reference's
+  // three-term recurrence and accumulation order)
   const bool rowk = c_.blk_rowkernel != 0 && B <= 32 && A.nrowblk > 0;
+  const bool clen = rowk && (c_.blk_rowkernel == 2 || (c_.blk_rowkernel < 0 && f32));
   auto run = [&](auto *tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
     T *x = (T *)xb[0], *y = (T *)xb[1];
@@ -370,6 +380,35 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
       pack_cols<T, double>(n_, B, g, reinterpret_cast<const double *const *>(colptr_.p), x,
                            c_.stream);
     }
+    T *yout = y;
+    if (clen && degmax >= 1) {
+      // Clenshaw sums from the top degree down: b_d = mu_d x + (2/dd)(A - cc)
+      // b_{d+1} - b_{d+2}; the d = 0 step (1/dd) is the filtered vector.
+      // Three rotating buffers; x (the input block) is read by every step.
+      T *ring[3] = {y, b[0], b[1]};
+      const T *b1 = nullptr, *b2 = nullptr;
+      int nr = 0;
+      for (int d = degmax; d >= 0; d--) {
+        T *out = ring[nr];
+        if ((const T *)out == b1 || (const T *)out == b2) {
+          nr = (nr + 1) % 3;
+          out = ring[nr];
+        }
+        nr = (nr + 1) % 3;
+        clen_step_blkrow<T>(A, B, b1, b2, x, out, cc, d == 0 ? 1.0 / dd : 2.0 / dd, mus_.p, d,
+                            c_.stream);
+        int nact = 0;  // columns whose series has reached degree d + 1
+        while (nact < g && degs[(size_t)nact] >= d + 1) nact++;
+        if (b1) {
+          c_.kstats.filter_bytes += (double)clen_launch_bytes(A, (int)elem, nact, b2 != nullptr);
+          c_.kstats.filter_steps += nact;
+          c_.kstats.filter_launches++;
+        }
+        b2 = b1;
+        b1 = out;
+        yout = out;
+      }
+    } else {
     // third work buffer: reuse x once it is no longer read (after step 2)
     const T *vk = x, *vkm1 = nullptr;
     T *bufs[3] = {b[0], b[1], x};
@@ -396,15 +435,16 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
       vkm1 = vk;
       vk = vkp1;
     }
+    }
     if constexpr (std::is_same<T, float>::value) {
       if (b32_)
-        unpack_cols<float, float>(n_, B, g, y, reinterpret_cast<float *const *>(colptr_.p + B),
-                                  c_.stream);
+        unpack_cols<float, float>(n_, B, g, yout,
+                                  reinterpret_cast<float *const *>(colptr_.p + B), c_.stream);
       else
-        unpack_cols<float, double>(n_, B, g, y,
+        unpack_cols<float, double>(n_, B, g, yout,
                                    reinterpret_cast<double *const *>(colptr_.p + B), c_.stream);
     } else {
-      unpack_cols<T, double>(n_, B, g, y, reinterpret_cast<double *const *>(colptr_.p + B),
+      unpack_cols<T, double>(n_, B, g, yout, reinterpret_cast<double *const *>(colptr_.p + B),
                              c_.stream);
     }
   };

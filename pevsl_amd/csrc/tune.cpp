@@ -47,8 +47,10 @@ double time_vec(Ctx &c, int lanes, int reps) {
   return ev.seconds() / reps;
 }
 
+// kind: 0 lane per (row, column), 1 row per thread, 2 row per thread in
+// Clenshaw form
 template <typename T>
-double time_blk(Ctx &c, int B, int reps, DBuf<double> &mus, bool rowk) {
+double time_blk(Ctx &c, int B, int reps, DBuf<double> &mus, int kind) {
   const DevCsr &A = *c.A;
   const size_t blk = (size_t)A.n * B;
   DBuf<T> b;
@@ -56,7 +58,9 @@ double time_blk(Ctx &c, int B, int reps, DBuf<double> &mus, bool rowk) {
   PV_HIP(hipMemsetAsync(b.p, 0, sizeof(T) * 4 * blk, c.stream));
   T *v0 = b.p, *v1 = b.p + blk, *v2 = b.p + 2 * blk, *y = b.p + 3 * blk;
   auto step = [&](const T *vk, const T *vkm1, T *vkp1) {
-    if (rowk)
+    if (kind == 2)
+      clen_step_blkrow<T>(A, B, vk, vkm1, y, vkp1, 6.0, 1.0 / 3, mus.p, 2, c.stream);
+    else if (kind == 1)
       cheb_step_blkrow<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
     else
       cheb_step_blk<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
@@ -111,8 +115,8 @@ int pEVSL_Autotune(pevsl_Data *p, int reps, pevsl_TuneResult *res) {
     // skip widths whose blocks would not fit comfortably (4 blocks)
     const double need = 4.0 * c.A->n * (double)B * elem;
     if (need > 32e9) break;
-    const double t = f32 ? time_blk<float>(c, B, reps, mus, false)
-                         : time_blk<double>(c, B, reps, mus, false);
+    const double t = f32 ? time_blk<float>(c, B, reps, mus, 0)
+                         : time_blk<double>(c, B, reps, mus, 0);
     res->t_block[i] = t;
     res->bytes_block[i] = step_bytes(*c.A, elem, B);
     if (t / B < bestc) {
@@ -121,13 +125,25 @@ int pEVSL_Autotune(pevsl_Data *p, int reps, pevsl_TuneResult *res) {
       rowk = 0;
     }
     if (B <= 32) {
-      const double tr = f32 ? time_blk<float>(c, B, reps, mus, true)
-                            : time_blk<double>(c, B, reps, mus, true);
+      const double tr = f32 ? time_blk<float>(c, B, reps, mus, 1)
+                            : time_blk<double>(c, B, reps, mus, 1);
       res->t_blockrow[i] = tr;
       if (tr / B < bestc) {
         bestc = tr / B;
         bw = B;
         rowk = 1;
+      }
+      // Clenshaw form: FP32 only (the FP64 path keeps the reference's
+      // recurrence)
+      if (f32) {
+        const double tc = time_blk<float>(c, B, reps, mus, 2);
+        res->t_blockclen[i] = tc;
+        res->bytes_clen[i] = step_bytes(*c.A, elem, B) - (double)c.A->n * B * elem;
+        if (tc / B < bestc) {
+          bestc = tc / B;
+          bw = B;
+          rowk = 2;
+        }
       }
     }
   }

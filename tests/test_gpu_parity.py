@@ -324,11 +324,14 @@ def test_tridiag_multisection_against_lapack(lib, n):
     assert abs(tr.value - sel.sum()) <= 64 * np.finfo(float).eps * scale * np.sqrt(n) * max(1, len(sel))
 
 
-@pytest.mark.parametrize("batch", [False, True])
-def test_mixed_precision_c1(lib, batch):
-    banner("C1 mixed precision (FP32 filter + FP64 Rayleigh-Ritz) batch=%s" % batch)
-    # stated mixed-path tolerance: residual < 1e-5, eigenvalue error < 1e-10
-    r = driver.lap_slicing_run(20, 20, 20, 0.6, 1.2, 5, batch=batch, prec=capi.PEVSL_PREC_FP32)
+@pytest.mark.parametrize("batch,rowk", [(False, -1), (True, 1), (True, 2)])
+def test_mixed_precision_c1(lib, batch, rowk):
+    banner("C1 mixed precision (FP32 filter + FP64 Rayleigh-Ritz) batch=%s block kernel %d"
+           % (batch, rowk))
+    # stated mixed-path tolerance: residual < 1e-5, eigenvalue error < 1e-10;
+    # block kernel 1 = three-term recurrence, 2 = Clenshaw form
+    r = driver.lap_slicing_run(20, 20, 20, 0.6, 1.2, 5, batch=batch, prec=capi.PEVSL_PREC_FP32,
+                               block_rowkernel=rowk)
     for s in r["slices"]:
         assert s["nev"] == s["nev_exact"]
         assert s["max_rel_err"] < 1e-10
