@@ -530,6 +530,16 @@ k_dot_part(int n, const double *__restrict__ x, const double *__restrict__ y,
   block_sum_store(acc, part + blockIdx.x);
 }
 
+// out[e] = sum_p part[p*nel + e], p in order (one thread per element)
+__global__ void k_sum_parts(int nparts, size_t nel, const double *__restrict__ part,
+                            double *__restrict__ out) {
+  const size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (e >= nel) return;
+  double v = 0.0;
+  for (int p = 0; p < nparts; p++) v += part[(size_t)p * nel + e];
+  out[e] = v;
+}
+
 __global__ void __launch_bounds__(kThreads)
 k_dot_part_f32(int n, const float *__restrict__ x, const float *__restrict__ y,
                double *__restrict__ part) {
@@ -1325,6 +1335,12 @@ void reduce_partials(const double *part, int nblk, int ncol, double *out,
   if (ncol <= 0) return;
   hipLaunchKernelGGL(k_reduce_partials, dim3(ncol), dim3(kThreads), 0, s, part,
                      nblk, ncol, out);
+}
+
+void sum_parts(const double *part, int nparts, size_t nel, double *out, hipStream_t s) {
+  if (nel == 0) return;
+  hipLaunchKernelGGL(k_sum_parts, dim3(grid_for(nel)), dim3(kThreads), 0, s, nparts, nel, part,
+                     out);
 }
 
 int dot_part_blocks(int n) { return lan_vec_blocks(n); }

@@ -181,6 +181,36 @@ void cheb_filter_col32(Ctx &c, const pevsl_Polparams *pol, const float *dv, floa
 
 namespace {
 
+// C (m1 x m2, ldc = m1) = A^T B for tall column-major A (n x m1) and B
+// (n x m2).  A single DGEMM with the long n as its inner dimension leaves most
+// of the chip idle at m1, m2 ~ 1e3; the rows are split into kGramParts
+// chunks computed as one strided-batched DGEMM (MFMA), whose partial
+// products are then summed in a fixed order.
+constexpr int kGramParts = 64;
+void gram_tn(rocblas_handle h, hipStream_t st, int n, int m1, const double *A, int lda, int m2,
+             const double *B, int ldb, double *C, DBuf<double> &work) {
+  const double one = 1.0, zero = 0.0;
+  const int chunk = n / kGramParts;
+  if (chunk < 1024) {
+    PV_BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, m1, m2, n,
+                          &one, A, lda, B, ldb, &zero, C, m1));
+    return;
+  }
+  const int rem = n - kGramParts * chunk;
+  const int parts = kGramParts + (rem > 0 ? 1 : 0);
+  const size_t blk = (size_t)m1 * m2;
+  double *wk = work.ensure(blk * parts);
+  PV_BLAS(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none,
+                                        m1, m2, chunk, &one, A, lda, chunk, B, ldb, chunk,
+                                        &zero, wk, m1, (rocblas_stride)blk, kGramParts));
+  if (rem > 0)
+    PV_BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, m1, m2, rem,
+                          &one, A + (size_t)kGramParts * chunk, lda,
+                          B + (size_t)kGramParts * chunk, ldb, &zero, wk + blk * kGramParts,
+                          m1));
+  sum_parts(wk, parts, blk, C, st);
+}
+
 class BatchSolver {
  public:
   BatchSolver(Ctx &c, pevsl_Data *p, int ns) : c_(c), p_(p), ns_(ns), S_(ns) {}
@@ -728,10 +758,11 @@ void BatchSolver::ritz_rr(int s, double tol) {
   H.ensure((size_t)nsel * nsel);
   th.ensure((size_t)nsel);
   ework.ensure((size_t)nsel);
-  PV_BLAS(rocblas_dgemm(c_.blas, rocblas_operation_transpose, rocblas_operation_none, nsel, nsel,
-                        n_, &one, U.p, (int)ld, U.p, (int)ld, &zero, G.p, nsel));
-  PV_BLAS(rocblas_dgemm(c_.blas, rocblas_operation_transpose, rocblas_operation_none, nsel, nsel,
-                        n_, &one, U.p, (int)ld, W.p, (int)ld, &zero, H.p, nsel));
+  {
+    DBuf<double> gw;
+    gram_tn(c_.blas, st, n_, nsel, U.p, (int)ld, nsel, U.p, (int)ld, G.p, gw);
+    gram_tn(c_.blas, st, n_, nsel, U.p, (int)ld, nsel, W.p, (int)ld, H.p, gw);
+  }
   PV_BLAS(rocsolver_dsygvd(c_.blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper,
                            nsel, H.p, nsel, G.p, nsel, th.p, ework.p, info.p));
   int hinfo = 0;
