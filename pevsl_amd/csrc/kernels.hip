@@ -941,8 +941,14 @@ __device__ int sturm_less(int m, const double *__restrict__ d,
 // ws: [0] lower bound, [1] upper bound, [2] pivmin, [3] first index >= thr
 // Gershgorin bounds by a block reduction, then one Sturm count at thr.
 __global__ void __launch_bounds__(kThreads)
-k_tri_prep(int m, const double *__restrict__ d, const double *__restrict__ e,
-           double thr, double *__restrict__ ws) {
+k_tri_prep(int m, size_t ld, const double *__restrict__ d, const double *__restrict__ e,
+           const double *__restrict__ thr_arr, double thr, double *__restrict__ ws) {
+  // problem blockIdx.x of a batch: T_p in d/e + p*ld, 4 workspace values each
+  const int pb = blockIdx.x;
+  d += (size_t)pb * ld;
+  e += (size_t)pb * ld;
+  ws += 4 * (size_t)pb;
+  if (thr_arr) thr = thr_arr[pb];
   double lo = HUGE_VAL, hi = -HUGE_VAL, emax2 = 0.0;
   for (int i = threadIdx.x; i < m; i += kThreads) {
     double r = 0.0;
@@ -986,10 +992,17 @@ k_tri_prep(int m, const double *__restrict__ d, const double *__restrict__ e,
 // point with count <= idx and the first with count > idx, so each round
 // shrinks the interval 65-fold (about 9 rounds to double precision).
 __global__ void __launch_bounds__(64)
-k_tri_msect(int m, const double *__restrict__ d, const double *__restrict__ e,
-            double thr, const double *__restrict__ ws, double *__restrict__ lam,
-            int all) {
+k_tri_msect(int m, size_t ld, const double *__restrict__ d, const double *__restrict__ e,
+            const double *__restrict__ thr_arr, double thr, const double *__restrict__ ws,
+            double *__restrict__ lam, int all) {
   const int idx = blockIdx.x;
+  // problem blockIdx.y of a batch
+  const int pb = blockIdx.y;
+  d += (size_t)pb * ld;
+  e += (size_t)pb * ld;
+  lam += (size_t)pb * ld;
+  ws += 4 * (size_t)pb;
+  if (thr_arr) thr = thr_arr[pb];
   const int first = all ? 0 : (int)ws[3];
   if (idx < first || idx >= m) return;
   const int l = threadIdx.x;
@@ -1017,10 +1030,14 @@ k_tri_msect(int m, const double *__restrict__ d, const double *__restrict__ e,
   if (l == 0) lam[idx] = 0.5 * (lo + hi);
 }
 
-__global__ void k_tri_sum(int m, const double *__restrict__ ws,
+__global__ void k_tri_sum(int m, size_t ld, const double *__restrict__ ws,
                           const double *__restrict__ lam,
                           double *__restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (threadIdx.x != 0) return;
+  const int pb = blockIdx.x;
+  ws += 4 * (size_t)pb;
+  lam += (size_t)pb * ld;
+  out += 2 * (size_t)pb;
   const int first = (int)ws[3];
   double s = 0.0;
   for (int i = first; i < m; i++) s += lam[i];
@@ -1498,21 +1515,33 @@ void tridiag_trace_above(int m, const double *d, const double *e, double thr,
                          double *lam, double *out, hipStream_t s) {
   // ws lives in out[2..5]
   double *ws = out + 2;
-  hipLaunchKernelGGL(k_tri_prep, dim3(1), dim3(kThreads), 0, s, m, d, e, thr, ws);
-  hipLaunchKernelGGL(k_tri_msect, dim3(m), dim3(64), 0, s, m, d, e, thr,
+  hipLaunchKernelGGL(k_tri_prep, dim3(1), dim3(kThreads), 0, s, m, (size_t)0, d, e,
+                     (const double *)nullptr, thr, ws);
+  hipLaunchKernelGGL(k_tri_msect, dim3(m, 1), dim3(64), 0, s, m, (size_t)0, d, e,
+                     (const double *)nullptr, thr, (const double *)ws, lam, 0);
+  hipLaunchKernelGGL(k_tri_sum, dim3(1), dim3(64), 0, s, m, (size_t)0, (const double *)ws,
+                     (const double *)lam, out);
+}
+
+void tridiag_trace_above_batch(int np, int m, size_t ld, const double *d, const double *e,
+                               const double *thr, double *lam, double *ws, double *out,
+                               hipStream_t s) {
+  if (np <= 0 || m <= 0) return;
+  hipLaunchKernelGGL(k_tri_prep, dim3(np), dim3(kThreads), 0, s, m, ld, d, e, thr, 0.0, ws);
+  hipLaunchKernelGGL(k_tri_msect, dim3(m, np), dim3(64), 0, s, m, ld, d, e, thr, 0.0,
                      (const double *)ws, lam, 0);
-  hipLaunchKernelGGL(k_tri_sum, dim3(1), dim3(64), 0, s, m,
-                     (const double *)ws, (const double *)lam, out);
+  hipLaunchKernelGGL(k_tri_sum, dim3(np), dim3(64), 0, s, m, ld, (const double *)ws,
+                     (const double *)lam, out);
 }
 
 void tridiag_eigvals(int m, const double *d, const double *e, double *lam,
                      hipStream_t s) {
   // uses lam[m..m+3] as workspace for the bounds: caller gives m+4 doubles
   double *ws = lam + m;
-  hipLaunchKernelGGL(k_tri_prep, dim3(1), dim3(kThreads), 0, s, m, d, e, -HUGE_VAL,
-                     ws);
-  hipLaunchKernelGGL(k_tri_msect, dim3(m), dim3(64), 0, s, m, d, e, -HUGE_VAL,
-                     (const double *)ws, lam, 1);
+  hipLaunchKernelGGL(k_tri_prep, dim3(1), dim3(kThreads), 0, s, m, (size_t)0, d, e,
+                     (const double *)nullptr, -HUGE_VAL, ws);
+  hipLaunchKernelGGL(k_tri_msect, dim3(m, 1), dim3(64), 0, s, m, (size_t)0, d, e,
+                     (const double *)nullptr, -HUGE_VAL, (const double *)ws, lam, 1);
 }
 
 void kpm_intchx(int Mdeg, const double *mu, int npts, const double *th,

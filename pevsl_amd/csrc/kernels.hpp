@@ -165,6 +165,12 @@ void lan_scale_new(const LanBatch &L, int k, hipStream_t s);
 // Workspace: lam (m doubles).
 void tridiag_trace_above(int m, const double *d, const double *e, double thr,
                          double *lam, double *out, hipStream_t s);
+// The same for np matrices of equal order m at once: T_p in d/e + p*ld,
+// thresholds thr[p] (device), eigenvalues in lam + p*ld, 4 workspace values
+// per problem in ws, out[2p] = sum and out[2p+1] = count.
+void tridiag_trace_above_batch(int np, int m, size_t ld, const double *d, const double *e,
+                               const double *thr, double *lam, double *ws, double *out,
+                               hipStream_t s);
 // All eigenvalues of T by bisection (ascending), for tests and small m.
 void tridiag_eigvals(int m, const double *d, const double *e, double *lam,
                      hipStream_t s);

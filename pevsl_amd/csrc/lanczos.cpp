@@ -241,7 +241,7 @@ class BatchSolver {
   void lanczos_update(const std::vector<int> &act, int k);
   void cgs(const std::vector<int> &act, int k);
   void breakdown(int s, int k);
-  void convergence_test(int s, int k);
+  void convergence_tests(const std::vector<int> &ts, int k);
   void ritz(int s, double tol);
   void ritz_rr(int s, double tol);
   void free_basis(int s);
@@ -265,6 +265,7 @@ class BatchSolver {
   DBuf<double> mus_;
   DBuf<void *> colptr_;
   DBuf<double> tri_;   // tridiagonal work
+  std::vector<double> tri_h_;
   double *pinned_ = nullptr;
   int *pinned_i_ = nullptr;
   std::vector<int> mus_key_;
@@ -565,31 +566,46 @@ void BatchSolver::breakdown(int s, int k) {
   PV_HIP(hipStreamSynchronize(c_.stream));
 }
 
-void BatchSolver::convergence_test(int s, int k) {
-  Slice &sl = S_[(size_t)s];
+// Restricted-trace stopping test (SRC/cheblanNr.c:311-333) for every slice
+// in ts at step k: the sum of the Ritz values of p(A) above bar by batched
+// Sturm multisection on the device, one launch sequence and one
+// synchronization for all slices.
+void BatchSolver::convergence_tests(const std::vector<int> &ts, int k) {
+  const int np = (int)ts.size();
+  if (np == 0) return;
   const int kdim = k + 1;
-  double *d = tri_.ensure((size_t)3 * kdim + 16);
-  double *e = d + kdim, *lam = e + kdim, *out = lam + kdim;
-  PV_HIP(hipMemcpyAsync(d, sl.dT.data(), sizeof(double) * kdim, hipMemcpyHostToDevice,
-                        c_.stream));
-  if (kdim > 1)
-    PV_HIP(hipMemcpyAsync(e, sl.eT.data(), sizeof(double) * (kdim - 1),
-                          hipMemcpyHostToDevice, c_.stream));
+  const size_t ld = (size_t)kdim;
+  double *d = tri_.ensure((size_t)np * (3 * ld + 8) + 16);
+  double *e = d + np * ld, *lam = e + np * ld, *ws = lam + np * ld, *out = ws + 4 * (size_t)np,
+         *thr = out + 2 * (size_t)np;
+  tri_h_.assign((size_t)np * (2 * ld + 1), 0.0);
+  for (int p = 0; p < np; p++) {
+    const Slice &sl = S_[(size_t)ts[(size_t)p]];
+    std::copy(sl.dT.begin(), sl.dT.begin() + kdim, tri_h_.begin() + (long)(p * ld));
+    std::copy(sl.eT.begin(), sl.eT.begin() + (kdim - 1),
+              tri_h_.begin() + (long)(np * ld + p * ld));
+    tri_h_[(size_t)2 * np * ld + p] = sl.bar - 5 * DBL_EPSILON;
+  }
   const double t0 = wtime();
-  tridiag_trace_above(kdim, d, e, sl.bar - 5 * DBL_EPSILON, lam, out, c_.stream);
-  PV_HIP(hipMemcpyAsync(pinned_, out, sizeof(double) * 2, hipMemcpyDeviceToHost,
+  PV_HIP(hipMemcpyAsync(d, tri_h_.data(), sizeof(double) * 2 * np * ld, hipMemcpyHostToDevice,
+                        c_.stream));
+  PV_HIP(hipMemcpyAsync(thr, tri_h_.data() + 2 * np * ld, sizeof(double) * np,
+                        hipMemcpyHostToDevice, c_.stream));
+  tridiag_trace_above_batch(np, kdim, ld, d, e, thr, lam, ws, out, c_.stream);
+  PV_HIP(hipMemcpyAsync(pinned_, out, sizeof(double) * 2 * np, hipMemcpyDeviceToHost,
                         c_.stream));
   PV_HIP(hipStreamSynchronize(c_.stream));
   p_->stats->t_eig += wtime() - t0;
   c_.kstats.t_eig += wtime() - t0;
-  sl.kdim = kdim;
-  sl.tr1 = pinned_[0];
-  sl.nconv = (int)pinned_[1];
-  if (fabs(sl.tr1 - sl.tr0) < 2e-12 || fabs(sl.tr1) + fabs(sl.tr0) < 1e-10) {
-    sl.done = true;
+  for (int p = 0; p < np; p++) {
+    Slice &sl = S_[(size_t)ts[(size_t)p]];
+    sl.kdim = kdim;
+    sl.tr1 = pinned_[2 * p];
+    sl.nconv = (int)pinned_[2 * p + 1];
+    if (fabs(sl.tr1 - sl.tr0) < 2e-12 || fabs(sl.tr1) + fabs(sl.tr0) < 1e-10) sl.done = true;
+    sl.tr0 = sl.tr1;
+    if (k == sl.maxit - 1) sl.done = true;
   }
-  sl.tr0 = sl.tr1;
-  if (k == sl.maxit - 1) sl.done = true;
 }
 
 // Ritz pairs of slice s and their acceptance (SRC/cheblanNr.c:339-413).
@@ -979,16 +995,21 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
       breakdown(s, k);
       betas[(size_t)s] = 0.0;
     }
+    std::vector<int> tests;
     for (int s : act) {
       Slice &sl = S_[(size_t)s];
       sl.eT[(size_t)k] = betas[(size_t)s];
       sl.its = k + 1;
       if ((k < kNtest || (k - kNtest) % kCycle != 0) && k != sl.maxit - 1) continue;
-      convergence_test(s, k);
-      if (fstats) {
-        fprintf(fstats, "  k %4d:   nconv %4d  tr1 %21.15e\n", k, sl.nconv, sl.tr1);
-        fflush(fstats);
+      tests.push_back(s);
+    }
+    convergence_tests(tests, k);
+    if (fstats) {
+      for (int s : tests) {
+        fprintf(fstats, "  k %4d:   nconv %4d  tr1 %21.15e\n", k, S_[(size_t)s].nconv,
+                S_[(size_t)s].tr1);
       }
+      fflush(fstats);
     }
     // retire converged slices: Ritz pairs, then release their basis
     std::vector<int> still;
