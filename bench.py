@@ -10,6 +10,16 @@ one slice, or --batch slices as columns of one block SpMM); step i solves
 slice (first + i*batch) mod nslices.  `value` = accepted eigenpairs per
 second over the timed steps.  --mode mixed runs the filter and KPM in FP32.
 
+After the timed FP64 steps the same slices are solved once more on the mixed
+path (FP32 filter in Clenshaw form, FP32 basis storage, FP64 Lanczos
+coefficients / reorthogonalization sums / Rayleigh-Ritz / residuals); the
+`mixed` object reports its rate and the ratio to FP64 (--no-mixed skips it).
+
+`roofline` is the dominant kernel of the timed region (largest event time):
+the fused filter step or one of the two CGS kernels.  Its `traffic` (HBM
+bytes per launch) comes from the hardware-counter summary committed in
+profiles/pmc_traffic.json, measured with pevsl_amd/profile_probe.py.
+
 Prints one JSON line (rank 0 of a single process).
 """
 import argparse
@@ -59,6 +69,20 @@ def cpu_baseline(grid, a, b, nev_slice, its, nev_found, t_budget=25.0):
             "t_step_s": t_step, "t_cgs512_s": t_cgs, "projected_slice_s": T}
 
 
+def load_traffic():
+    """Per-launch HBM bytes of the hot kernels from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(p) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    out = {k: v.get("hbm_bytes_per_launch") for k, v in d.get("kernels", {}).items()}
+    out["_source"] = "profiles/pmc_traffic.json (%s)" % d.get("workload", "")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +95,7 @@ def main():
     ap.add_argument("--batch", type=int, default=1)
     ap.add_argument("--nvec", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (multi-GPU sharding is "
@@ -108,7 +133,7 @@ def main():
     log("setup %.2fs (kpm+slicing %.2fs), ecount %.1f, lanes %d, degs %s" % (
         t_setup, t_kpm, ecount, tune.lanes, [p.deg for p in pols]))
 
-    def step(i):
+    def step(i, inst=inst):
         sl = [(args.first_slice + (i * args.batch) + j) % args.nslices for j in range(args.batch)]
         intvs = [[sli[s], sli[s + 1], lmin, lmax] for s in sl]
         rc, outs = driver.cheblan_batch(inst, intvs, [mlan] * len(sl), tol, vinit,
@@ -140,17 +165,56 @@ def main():
 
     cgs_bw = ks["cgs_bytes"] / ks["t_cgs"] / 1e9 if ks["t_cgs"] > 0 else 0.0
     filt_bw = ks["filter_bytes"] / ks["t_filter"] / 1e9 if ks["t_filter"] > 0 else 0.0
-    launches = max(1, ks["cgs_passes"])
-    roof = {"bound": "hbm", "kernel": "CGS-DGKS pass (k_lan_gemvt + k_lan_gemvn)",
-            "achieved": round(cgs_bw, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(cgs_bw / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_pass_avg": ks["cgs_bytes"] / launches,
-            "sec_per_pass_avg": ks["t_cgs"] / launches}
-    filt = {"bound": "hbm", "kernel": "fused Chebyshev step (k_cheb_vec/k_cheb_blk)",
+    passes = max(1, ks["cgs_passes"])
+    flaunch = max(1, ks["filter_launches"])
+    traffic = load_traffic()
+    cgs = {"bound": "hbm", "kernel": "CGS-DGKS pass (k_lan_gemvt + k_lan_gemvn)",
+           "achieved": round(cgs_bw, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(cgs_bw / HBM_PEAK_GBS, 4),
+           "traffic": traffic.get("cgs_pass_fp64_per_column"),
+           "bytes_per_pass_avg": ks["cgs_bytes"] / passes,
+           "sec_per_pass_avg": ks["t_cgs"] / passes}
+    fkern = "k_cheb_stream<double>" if (prec == capi.PEVSL_PREC_FP64 and args.batch == 1) \
+        else "block filter kernels"
+    filt = {"bound": "hbm", "kernel": "fused Chebyshev step (%s)" % fkern,
             "achieved": round(filt_bw, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(filt_bw / HBM_PEAK_GBS, 4),
+            "traffic": traffic.get("k_cheb_stream<double>") if fkern.startswith("k_cheb") else None,
+            "bytes_per_launch": ks["filter_bytes"] / flaunch,
+            "sec_per_launch_avg": ks["t_filter"] / flaunch,
             "launches": ks["filter_launches"], "column_steps": ks["filter_steps"],
             "t_filter_s": ks["t_filter"]}
+    # the dominant single kernel: the filter step, unless one CGS kernel (about
+    # half of a pass) takes longer
+    roof = dict(filt) if ks["t_filter"] >= 0.5 * ks["t_cgs"] else dict(cgs)
+    roof["traffic_source"] = traffic.get("_source")
+    mixed = None
+    if not args.no_mixed:
+        instm = driver.Instance(lib, M, prec=capi.PEVSL_PREC_FP32,
+                                basis_prec=capi.PEVSL_PREC_FP32)
+        tm = capi.TuneResult()
+        lib.pEVSL_Autotune(instm.p, 20, C.byref(tm))
+        if args.batch == 1:
+            lib.pEVSL_SetSpmvTuning(instm.p, tm.lanes, 1)
+        t1 = time.time()
+        mres = []
+        for i in range(args.steps):
+            mres += step(i, instm)
+        tmix = time.time() - t1
+        ksm = instm.kernel_stats()
+        instm.close()
+        mfound = sum(r["found"] for r in mres)
+        mixed = {"value": round(mfound / tmix, 3), "unit": "eigenpairs/s",
+                 "seconds": round(tmix, 2), "speedup_vs_fp64": round((mfound / tmix) / value, 3),
+                 "found": mfound, "exact": sum(r["exact"] for r in mres),
+                 "max_rel_err": max((r["max_rel_err"] or 0.0) for r in mres),
+                 "max_residual": max((r["max_res"] or 0.0) for r in mres),
+                 "tolerance": "residual < 1e-5 (FP32-filtered basis), eigenvalue rel err < 1e-10",
+                 "phase_s": {"filter": round(ksm["t_filter"], 3), "cgs": round(ksm["t_cgs"], 3),
+                             "eig": round(ksm["t_eig"], 3), "ritz": round(ksm["t_ritz"], 3)},
+                 "filter_GBs": round(ksm["filter_bytes"] / max(ksm["t_filter"], 1e-9) / 1e9, 1),
+                 "cgs_GBs": round(ksm["cgs_bytes"] / max(ksm["t_cgs"], 1e-9) / 1e9, 1),
+                 "autotune": {"block_width": tm.block_width, "block_kernel": tm.block_rowkernel}}
     cpu = None
     if not args.no_cpu_baseline:
         r0 = results[0]
@@ -179,9 +243,11 @@ def main():
         "setup_s": round(t_setup, 3), "kpm_slicing_s": round(t_kpm, 3),
         "ecount": ecount,
         "phase_s": {"filter": round(ks["t_filter"], 3), "reorth": round(ks["t_reorth"], 3),
-                    "cgs": round(ks["t_cgs"], 3)},
+                    "cgs": round(ks["t_cgs"], 3), "eig": round(ks["t_eig"], 3),
+                    "ritz": round(ks["t_ritz"], 3)},
         "autotune": {"lanes": tune.lanes, "block_width": tune.block_width},
-        "roofline": roof, "filter_roofline": filt, "cpu_baseline": cpu,
+        "roofline": roof, "filter_roofline": filt, "cgs_roofline": cgs, "mixed": mixed,
+        "cpu_baseline": cpu,
     }
     for p in pols:
         lib.pEVSL_FreePol(C.byref(p))

@@ -169,6 +169,13 @@ typedef struct {
 } pevsl_TuneResult;
 int pEVSL_Autotune(pevsl_Data *pevsl, int reps, pevsl_TuneResult *res);
 
+/* Times `reps` launches of a plain copy kernel over n doubles (read n,
+ * write n; 16-byte accesses) on the current device: seconds per launch and
+ * its bytes (16 n).  A bandwidth reference for the other kernels and the
+ * known-size kernel used to calibrate hardware byte counters.  No
+ * counterpart in the reference. */
+int pEVSL_BenchCopy(size_t n, int reps, double *sec, double *bytes);
+
 /* Times `reps` classical Gram-Schmidt passes of one vector against a basis
  * of k FP64 columns (the reorthogonalization kernels); returns seconds per
  * pass and its algorithmic bytes (2 reads of the basis + vector traffic). */

@@ -241,6 +241,7 @@ SIGNATURES = {
                                         c_double_p, c_int_p, c_double_p]),
     "pEVSL_Autotune": (C.c_int, [DataP, C.c_int, C.POINTER(TuneResult)]),
     "pEVSL_BenchReorth": (C.c_int, [DataP, C.c_int, C.c_int, c_double_p, c_double_p]),
+    "pEVSL_BenchCopy": (C.c_int, [C.c_size_t, C.c_int, c_double_p, c_double_p]),
     # learned slicer
     "pEVSL_MlpCreate": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.c_ulonglong, C.POINTER(C.c_void_p)]),

@@ -530,6 +530,13 @@ k_dot_part(int n, const double *__restrict__ x, const double *__restrict__ y,
   block_sum_store(acc, part + blockIdx.x);
 }
 
+// y = x over n2 16-byte pairs (a plain copy: reference kernel for bandwidth
+// and for calibrating hardware byte counters)
+__global__ void k_copy2(size_t n2, const double2 *__restrict__ x, double2 *__restrict__ y) {
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n2) y[i] = x[i];
+}
+
 // out[e] = sum_p part[p*nel + e], p in order (one thread per element)
 __global__ void k_sum_parts(int nparts, size_t nel, const double *__restrict__ part,
                             double *__restrict__ out) {
@@ -1352,6 +1359,13 @@ void reduce_partials(const double *part, int nblk, int ncol, double *out,
   if (ncol <= 0) return;
   hipLaunchKernelGGL(k_reduce_partials, dim3(ncol), dim3(kThreads), 0, s, part,
                      nblk, ncol, out);
+}
+
+void copy_f64(size_t n, const double *x, double *y, hipStream_t s) {
+  const size_t n2 = n / 2;
+  if (n2)
+    hipLaunchKernelGGL(k_copy2, dim3(grid_for(n2)), dim3(kThreads), 0, s, n2,
+                       reinterpret_cast<const double2 *>(x), reinterpret_cast<double2 *>(y));
 }
 
 void sum_parts(const double *part, int nparts, size_t nel, double *out, hipStream_t s) {

@@ -89,6 +89,8 @@ int kpm_part_blocks(int n, int B);
 // out[c] = sum_b part[b*ncol + c], b < nblk (fixed order); out may be strided.
 void reduce_partials(const double *part, int nblk, int ncol, double *out,
                      hipStream_t s);
+// y = x for n doubles (n even, 16-byte aligned): plain copy kernel
+void copy_f64(size_t n, const double *x, double *y, hipStream_t s);
 // out[e] = sum_p part[p*nel + e] over p = 0..nparts-1 in order
 void sum_parts(const double *part, int nparts, size_t nel, double *out, hipStream_t s);
 // part[blk] = sum x_i*y_i over the block's rows; returns nblk

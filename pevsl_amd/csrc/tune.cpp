@@ -157,6 +157,29 @@ int pEVSL_Autotune(pevsl_Data *p, int reps, pevsl_TuneResult *res) {
   return 0;
 }
 
+int pEVSL_BenchCopy(size_t n, int reps, double *sec, double *bytes) {
+  if (n < 2 || reps < 1 || !sec || !bytes) return -1;
+  n &= ~(size_t)1;
+  hipStream_t s;
+  PV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  {
+    DBuf<double> x, y;
+    x.ensure(n);
+    y.ensure(n);
+    rand_fill(n, x.p, 11, 0, 0, s);
+    copy_f64(n, x.p, y.p, s);  // warm-up
+    EvTimer ev;
+    ev.start(s);
+    for (int r = 0; r < reps; r++) copy_f64(n, x.p, y.p, s);
+    ev.stop(s);
+    *sec = ev.seconds() / reps;
+    *bytes = 16.0 * (double)n;
+    PV_HIP(hipStreamSynchronize(s));
+  }
+  PV_HIP(hipStreamDestroy(s));
+  return 0;
+}
+
 int pEVSL_BenchReorth(pevsl_Data *p, int k, int reps, double *sec, double *bytes) {
   Ctx &c = ctx_of(p);
   const int n = p->n;
