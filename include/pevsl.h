@@ -57,8 +57,8 @@ void pEVSL_SpslicerLan(double *xi, double *yi, int n_int, int npts,
                        double *sli);
 
 /* ---- lantrbnd.c (INC/pevsl.h:28-30) --------------------------------------
- * Spectrum bounds by thick-restart Lanczos: not provided by this library;
- * returns PEVSL_NOT_IMPLEMENT.                         SRC/lantrbnd.c:31 */
+ * Spectrum bounds by thick-restart Lanczos (standard problem; the
+ * generalized one returns PEVSL_NOT_IMPLEMENT).        SRC/lantrbnd.c:31 */
 int pEVSL_LanTrbounds(pevsl_Data *pevsl, int lanm, int maxit, double tol,
                       pevsl_Parvec *vinit, int bndtype, double *lammin,
                       double *lammax, FILE *fstats);

@@ -900,3 +900,206 @@ void orc_spslicer_lan(const double *xi, double *yi, int n_int, int npts,
   }
   sli[n_int] = xi[npts - 1];
 }
+
+/* ------------------------------------------------------------------------
+ * Dense symmetric eigenproblem (oracle of SymEigenSolver, SRC/miscla.c,
+ * LAPACK DSYEV in the reference): Householder reduction to tridiagonal form
+ * Q^T A Q = T (Golub & Van Loan, Alg. 8.3.1), then the implicit QL solver
+ * above on T, eigenvectors Z = Q Z_T.  A and Z are n x n column-major;
+ * eigenvalues ascending.
+ * --------------------------------------------------------------------- */
+int orc_sym_eig(int n, const double *A, double *w, double *Z) {
+  if (n <= 0) return 0;
+  const size_t nn = (size_t)n * n;
+  double *B = (double *)xmalloc(sizeof(double) * nn);
+  double *Q = (double *)xmalloc(sizeof(double) * nn);
+  double *u = (double *)xmalloc(sizeof(double) * (size_t)n);
+  double *p = (double *)xmalloc(sizeof(double) * (size_t)n);
+  double *d = (double *)xmalloc(sizeof(double) * (size_t)n);
+  double *e = (double *)xmalloc(sizeof(double) * (size_t)n);
+  double *zt = (double *)xmalloc(sizeof(double) * nn);
+  memcpy(B, A, sizeof(double) * nn);
+  memset(Q, 0, sizeof(double) * nn);
+  for (int i = 0; i < n; i++) Q[(size_t)i * n + i] = 1.0;
+#define EL(M, r, c) (M)[(size_t)(c) * n + (r)]
+  for (int j = 0; j < n - 2; j++) {
+    /* Householder vector u for B(j+1:n, j): (I - 2uu^T/u^Tu) x = -+|x| e1 */
+    double nrm = 0.0;
+    for (int r = j + 1; r < n; r++) nrm += EL(B, r, j) * EL(B, r, j);
+    nrm = sqrt(nrm);
+    if (nrm == 0.0) continue;
+    const double x0 = EL(B, j + 1, j);
+    const double alpha = x0 >= 0 ? -nrm : nrm;
+    for (int r = 0; r < n; r++) u[r] = 0.0;
+    for (int r = j + 1; r < n; r++) u[r] = EL(B, r, j);
+    u[j + 1] -= alpha;
+    double uu = 0.0;
+    for (int r = j + 1; r < n; r++) uu += u[r] * u[r];
+    if (uu == 0.0) continue;
+    const double beta = 2.0 / uu;
+    /* B <- H B H with H = I - beta u u^T: p = beta B u, K = beta/2 u^T p,
+       q = p - K u, B -= u q^T + q u^T */
+    for (int r = 0; r < n; r++) {
+      double t = 0.0;
+      for (int c = j + 1; c < n; c++) t += EL(B, r, c) * u[c];
+      p[r] = beta * t;
+    }
+    double K = 0.0;
+    for (int r = j + 1; r < n; r++) K += u[r] * p[r];
+    K *= 0.5 * beta;
+    for (int r = 0; r < n; r++) p[r] -= K * u[r];
+    for (int c = 0; c < n; c++)
+      for (int r = 0; r < n; r++) EL(B, r, c) -= u[r] * p[c] + p[r] * u[c];
+    /* Q <- Q H */
+    for (int r = 0; r < n; r++) {
+      double t = 0.0;
+      for (int c = j + 1; c < n; c++) t += EL(Q, r, c) * u[c];
+      t *= beta;
+      for (int c = j + 1; c < n; c++) EL(Q, r, c) -= t * u[c];
+    }
+  }
+  for (int i = 0; i < n; i++) d[i] = EL(B, i, i);
+  for (int i = 0; i < n - 1; i++) e[i] = EL(B, i + 1, i);
+  const int info = orc_tridiag_eig(n, d, e, w, zt);
+  if (Z) {
+    for (int c = 0; c < n; c++)
+      for (int r = 0; r < n; r++) {
+        double t = 0.0;
+        for (int k = 0; k < n; k++) t += EL(Q, r, k) * zt[(size_t)c * n + k];
+        EL(Z, r, c) = t;
+      }
+  }
+#undef EL
+  free(B); free(Q); free(u); free(p); free(d); free(e); free(zt);
+  return info;
+}
+
+/* ------------------------------------------------------------------------
+ * Thick-restart Lanczos bounds of the spectrum: SRC/lantrbnd.c:31-408
+ * (standard problem).  Restart dimension lanm, at most maxit products with
+ * A, two Ritz pairs kept at each restart; bounds Ritz value -+ |spike|
+ * (bndtype <= 1) or the Kato-Temple form (bndtype 2); stop when
+ * t1 + t2 < tol (|lmin| + |lmax|).  vinit is not modified; lucky breakdowns
+ * draw from the libc rand() stream.
+ * --------------------------------------------------------------------- */
+int orc_lan_trbounds(const orc_csr *A, int lanm, int maxit, double tol, const double *vinit,
+                     int bndtype, double *lammin, double *lammax, int *its) {
+  const int n = A->nrows;
+  if (lanm > n) lanm = n;
+  const int lanm1 = lanm + 1;
+  if (lanm == n && maxit > n) maxit = n;
+  const int min_inner_step = 5;
+  const size_t ld = (size_t)n, T1 = (size_t)lanm1;
+  double *V = (double *)xmalloc(sizeof(double) * ld * (size_t)lanm1);
+  double *T = (double *)calloc(T1 * T1, sizeof(double));
+  double *Tk = (double *)xmalloc(sizeof(double) * T1 * T1);
+  double *Ev = (double *)xmalloc(sizeof(double) * T1 * T1);
+  double *Rval = (double *)xmalloc(sizeof(double) * T1);
+  double *R0 = (double *)xmalloc(sizeof(double) * ld);
+  double *R1 = (double *)xmalloc(sizeof(double) * ld);
+  double *work = (double *)xmalloc(sizeof(double) * 3 * T1);
+  double s[3] = {0, 0, 0}, lmin = 0.0, lmax = 0.0, t1 = 0.0, t2 = 0.0;
+  int it = 0, trlen = 0, rc = 0;
+  memcpy(V, vinit, sizeof(double) * ld);
+  double t = sqrt(dot(n, V, V));
+  for (int i = 0; i < n; i++) V[i] /= t;
+  while (it < maxit) {
+    double wn = 0.0, beta = 0.0;
+    int nwn = 0;
+    int k = trlen > 0 ? trlen + 1 : 0;
+    if (k + min_inner_step > lanm1) { rc = -1; break; }
+    if (trlen > 0) {
+      const int k1 = k - 1;
+      double *v = V + (size_t)k1 * ld, *vnew = V + (size_t)(k1 + 1) * ld;
+      orc_csrmv(A, v, vnew);
+      for (int i = 0; i < trlen; i++) {
+        T[(size_t)i * T1 + i] = Rval[i];
+        wn += fabs(Rval[i]);
+      }
+      s[k1] = dot(n, v, vnew);
+      for (int j = 0; j < k; j++) axpy(n, -s[j], V + (size_t)j * ld, vnew);
+      for (int i = 0; i < k1; i++) {
+        T[(size_t)trlen * T1 + i] = s[i];
+        T[(size_t)i * T1 + trlen] = s[i];
+        wn += 2.0 * fabs(s[i]);
+      }
+      T[(size_t)trlen * T1 + trlen] = s[k1];
+      wn += fabs(s[k1]);
+      beta = sqrt(dot(n, vnew, vnew));
+      wn += 2.0 * beta;
+      nwn += 3 * k;
+      if (beta * nwn < 1e-14 * wn) {
+        orc_rand_vec(n, vnew);
+        orc_cgs_dgks(n, k, 2, V, (int)ld, vnew, &beta, work);
+        for (int i = 0; i < n; i++) vnew[i] /= beta;
+        beta = 0.0;
+      } else {
+        for (int i = 0; i < n; i++) vnew[i] /= beta;
+      }
+      T[(size_t)k1 * T1 + k] = beta;
+      T[(size_t)k * T1 + k1] = beta;
+    }
+    while (k < lanm && it < maxit) {
+      k++;
+      double *v = V + (size_t)(k - 1) * ld, *vnew = V + (size_t)k * ld;
+      const double *zold = k > 1 ? V + (size_t)(k - 2) * ld : NULL;
+      orc_csrmv(A, v, vnew);
+      it++;
+      if (zold) axpy(n, -beta, zold, vnew);
+      const double alpha = dot(n, v, vnew);
+      T[(size_t)(k - 1) * T1 + (k - 1)] = alpha;
+      wn += fabs(alpha);
+      axpy(n, -alpha, v, vnew);
+      orc_cgs_dgks(n, k, 2, V, (int)ld, vnew, &beta, work);
+      wn += 2.0 * beta;
+      nwn += 3;
+      if (beta * nwn < 1e-14 * wn) {
+        orc_rand_vec(n, vnew);
+        orc_cgs_dgks(n, k, 2, V, (int)ld, vnew, &beta, work);
+        for (int i = 0; i < n; i++) vnew[i] /= beta;
+        beta = 0.0;
+      } else {
+        for (int i = 0; i < n; i++) vnew[i] /= beta;
+      }
+      T[(size_t)k * T1 + (k - 1)] = beta;
+      T[(size_t)(k - 1) * T1 + k] = beta;
+    }
+    /* eigenpairs of T(0:k, 0:k) (column-major with leading dimension k) */
+    for (int c = 0; c < k; c++)
+      for (int r = 0; r < k; r++) Tk[(size_t)c * k + r] = T[(size_t)c * T1 + r];
+    orc_sym_eig(k, Tk, Rval, Ev);
+    s[0] = beta * Ev[(size_t)0 * k + (k - 1)];
+    s[1] = beta * Ev[(size_t)(k - 1) * k + (k - 1)];
+    if (bndtype <= 1) {
+      t1 = fabs(s[0]);
+      t2 = fabs(s[1]);
+    } else {
+      t1 = 2.0 * s[0] * s[0] / (Rval[1] - Rval[0]);
+      t2 = 2.0 * s[1] * s[1] / (Rval[k - 1] - Rval[k - 2]);
+    }
+    lmin = Rval[0] - t1;
+    lmax = Rval[k - 1] + t2;
+    /* the two extreme Ritz vectors become V(:,0), V(:,1) */
+    for (size_t i = 0; i < ld; i++) {
+      double a0 = 0.0, a1 = 0.0;
+      for (int j = 0; j < k; j++) {
+        a0 += V[(size_t)j * ld + i] * Ev[(size_t)0 * k + j];
+        a1 += V[(size_t)j * ld + i] * Ev[(size_t)(k - 1) * k + j];
+      }
+      R0[i] = a0;
+      R1[i] = a1;
+    }
+    trlen = 2;
+    memcpy(V, R0, sizeof(double) * ld);
+    memcpy(V + ld, R1, sizeof(double) * ld);
+    Rval[1] = Rval[k - 1];
+    if (t1 + t2 < tol * (fabs(lmin) + fabs(lmax))) break;
+    memset(T, 0, sizeof(double) * T1 * T1);
+    memcpy(V + (size_t)trlen * ld, V + (size_t)k * ld, sizeof(double) * ld);
+  }
+  *lammin = lmin;
+  *lammax = lmax;
+  if (its) *its = it;
+  free(V); free(T); free(Tk); free(Ev); free(Rval); free(R0); free(R1); free(work);
+  return rc;
+}

@@ -80,6 +80,11 @@ int  orc_tridiag_eig(int n, const double *d, const double *e, double *w,
                      double *z);
 void orc_cgs_dgks(int n, int k, int i_max, const double *V, int ld, double *v,
                   double *nrmv, double *work);
+/* dense symmetric eigenpairs (A, Z column-major n x n; w ascending) */
+int  orc_sym_eig(int n, const double *A, double *w, double *Z);
+/* thick-restart Lanczos spectrum bounds (SRC/lantrbnd.c:31-408) */
+int  orc_lan_trbounds(const orc_csr *A, int lanm, int maxit, double tol, const double *vinit,
+                      int bndtype, double *lammin, double *lammax, int *its);
 
 /* KPM DOS + slicing */
 int  orc_dampcf(int m, int damping, double *jac);

@@ -62,6 +62,9 @@ def lib():
         L.orc_kpmdos.argtypes = [C.POINTER(OrcCsr), C.c_int, C.c_int, C.c_int, dp,
                                  dp, dp]
         L.orc_intchx.argtypes = [C.c_int, dp, C.c_int, dp, dp]
+        L.orc_sym_eig.argtypes = [C.c_int, dp, dp, dp]
+        L.orc_lan_trbounds.argtypes = [C.POINTER(OrcCsr), C.c_int, C.c_int, C.c_double, dp,
+                                       C.c_int, dp, dp, ip]
         L.orc_spslicer_kpm.argtypes = [dp, dp, C.c_int, dp, C.c_int, C.c_int]
         L.orc_set_pol_def.argtypes = [C.POINTER(OrcPol)]
         L.orc_find_pol.argtypes = [dp, C.POINTER(OrcPol)]
@@ -144,6 +147,26 @@ def kpmdos(A, Mdeg, damping, nvec, intv):
     iv = np.ascontiguousarray(intv, dtype=np.float64)
     rc = lib().orc_kpmdos(C.byref(A.m), Mdeg, damping, nvec, P(iv), P(mu), C.byref(e))
     return rc, mu, e.value
+
+
+def sym_eig(A):
+    """Eigenvalues (ascending) and eigenvectors of a dense symmetric matrix."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    n = A.shape[0]
+    w = np.empty(n)
+    Z = np.empty((n, n), order="F")
+    rc = lib().orc_sym_eig(n, A.ctypes.data_as(dp), w.ctypes.data_as(dp), Z.ctypes.data_as(dp))
+    return rc, w, Z
+
+
+def lan_trbounds(A, lanm, maxit, tol, vinit, bndtype=1):
+    """Thick-restart Lanczos spectrum bounds; returns (rc, lmin, lmax, its)."""
+    v = np.ascontiguousarray(vinit, dtype=np.float64)
+    lo, hi = C.c_double(), C.c_double()
+    its = C.c_int()
+    rc = lib().orc_lan_trbounds(C.byref(A.m), lanm, maxit, tol, P(v), bndtype, C.byref(lo),
+                                C.byref(hi), C.byref(its))
+    return rc, lo.value, hi.value, its.value
 
 
 def intchx(mu, xi):

@@ -207,11 +207,6 @@ int pEVSL_SetSigmaMult(pevsl_Data *p, double mult) {
   return 0;
 }
 
-int pEVSL_LanTrbounds(pevsl_Data *, int, int, double, pevsl_Parvec *, int,
-                      double *, double *, FILE *) {
-  return PEVSL_NOT_IMPLEMENT;
-}
-
 // ---------------------------------------------------------------------------
 // Extensions (pevsl_amd.h)
 // ---------------------------------------------------------------------------

@@ -138,6 +138,22 @@ class Instance:
                                    mu.ctypes.data_as(capi.c_double_p), C.byref(ecnt))
         return rc, mu, ecnt.value
 
+    def lan_trbounds(self, lanm=50, maxit=200, tol=1e-8, vinit=None, bndtype=1):
+        """pEVSL_LanTrbounds: (rc, lmin, lmax) by thick-restart Lanczos.
+        vinit defaults to a libc-rand() vector (pEVSL_ParvecRand)."""
+        v = vinit if vinit is not None else rand_vinit(self.lib, self.n_rows())
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        pv = capi.Parvec()
+        self.lib.pEVSL_ParvecCreateShell(len(vv), len(vv), 0, 0, C.byref(pv),
+                                         vv.ctypes.data_as(capi.c_double_p))
+        lo, hi = C.c_double(), C.c_double()
+        rc = self.lib.pEVSL_LanTrbounds(self.p, lanm, maxit, tol, C.byref(pv), bndtype,
+                                        C.byref(lo), C.byref(hi), None)
+        return rc, lo.value, hi.value
+
+    def n_rows(self):
+        return self.p.contents.n
+
     def landos(self, nvec, msteps, npts, intv):
         xd = np.zeros(npts)
         yd = np.zeros(npts)
@@ -270,7 +286,7 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
                     sli_override=None, block_width=0, log=None, verbose=False,
                     maxit_cap=None, lanes=0, ritz_refine=-1, mixed_tol=None,
                     block_rowkernel=-1, mlp=None, judge=True, judge_tol=0.15, wave=None,
-                    basis_prec=capi.PEVSL_PREC_FP64):
+                    basis_prec=capi.PEVSL_PREC_FP64, lanczos_bounds=False):
     """End-to-end run of the TESTS/Lap/LapPLanN.c flow on the GPU.  Returns a
     dict with the DOS, slices, per-slice filters, eigenvalues and timings.
 
@@ -282,7 +298,9 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
 
     With `batch`, slices are solved `wave` at a time (all at once when None)
     as the columns of one pEVSL_ChebLanNrBatch call, so that the Lanczos
-    bases of one wave fit in HBM."""
+    bases of one wave fit in HBM.  With `lanczos_bounds` the frame
+    [lmin, lmax] comes from pEVSL_LanTrbounds (50-vector restart, 1e-8)
+    instead of the hand-set Laplacian bounds."""
     lib = capi.load()
     t0 = time.time()
     ia, ja, av = laplacian3d(nx, ny, nz)
@@ -296,6 +314,12 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
                     basis_prec=basis_prec)
     out = {"n": n, "nnz": M.nnz, "t_setup": time.time() - t0}
     try:
+        if lanczos_bounds:
+            tb = time.time()
+            rc, lmin, lmax = inst.lan_trbounds(50, 3000, 1e-8)
+            if rc:
+                raise RuntimeError("LanTrbounds error %d" % rc)
+            out["bounds"] = {"lmin": lmin, "lmax": lmax, "seconds": time.time() - tb}
         xintv = np.array([a, b, lmin, lmax])
         t1 = time.time()
         if dostype == 0:

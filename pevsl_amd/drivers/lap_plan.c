@@ -12,8 +12,13 @@
  * (pEVSL_SlicerJudge) and the driver falls back to pEVSL_SpslicerKpm when
  * the largest full-KPM slice count exceeds the mean by more than 15%.
  *
+ * With -bounds 1 the spectrum bounds [lmin, lmax] come from
+ * pEVSL_LanTrbounds (thick-restart Lanczos, as TESTS/Gen_Lap/LapPLanN.c:138
+ * and TESTS/MM/MMPLanN.c:111 do) instead of the Laplacian's hand-set
+ * [0, 12] (TESTS/Lap/LapPLanN.c:86-87).
+ *
  * usage: lap_plan [-nx 20 -ny 20 -nz 20 -a 0.6 -b 1.2 -nslices 5
- *                  -mixed 0 -batch 0 -mlp weights.bin]
+ *                  -mixed 0 -batch 0 -mlp weights.bin -bounds 0]
  */
 #include <math.h>
 #include <stdio.h>
@@ -98,6 +103,18 @@ int main(int argc, char **argv) {
   pEVSL_SetAParcsr(pevsl, &A);
   if (mixed) pEVSL_SetFilterPrecision(pevsl, PEVSL_PREC_FP32);
 
+  if (argi(argc, argv, "bounds", 0)) {
+    pevsl_Parvec vb;
+    pEVSL_ParvecCreate(A.ncol_global, A.ncol_local, A.first_col, MPI_COMM_WORLD, &vb);
+    pEVSL_ParvecRand(&vb);
+    double t0b = pEVSL_Wtime();
+    ierr = pEVSL_LanTrbounds(pevsl, 50, 3000, 1e-8, &vb, 1, &lmin, &lmax, NULL);
+    pEVSL_ParvecFree(&vb);
+    if (ierr) { printf("LanTrbounds error %d\n", ierr); return 1; }
+    printf("LanTrbounds: [%.10f, %.10f] (%.3f s)\n", lmin, lmax, pEVSL_Wtime() - t0b);
+    xintv[2] = lmin;
+    xintv[3] = lmax;
+  }
   double *mu = (double *)malloc((Mdeg + 1) * sizeof(double));
   double *sli = (double *)malloc((nslices + 1) * sizeof(double));
   const char *mlp_path = args(argc, argv, "mlp");

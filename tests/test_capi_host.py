@@ -192,5 +192,6 @@ def test_coo_csr_roundtrip_and_sortrow(lib):
 def test_generalized_setters_report_not_implemented(lib):
     assert lib.pEVSL_SetGenEig(None) == capi.PEVSL_NOT_IMPLEMENT
     assert lib.pEVSL_SetBSol(None, None, None) == capi.PEVSL_NOT_IMPLEMENT
-    assert lib.pEVSL_LanTrbounds(None, 10, 10, 1e-8, None, 0, None, None, None) == \
-        capi.PEVSL_NOT_IMPLEMENT
+    # LanTrbounds is implemented (standard problem); missing arguments are an
+    # error, not a crash
+    assert lib.pEVSL_LanTrbounds(None, 10, 10, 1e-8, None, 0, None, None, None) == -1

@@ -391,3 +391,67 @@ def test_stream_kernel_end_to_end(lib):
     for s, ds in zip(r["slices"], d["slices"]):
         assert s["nev"] == s["nev_exact"] == ds["nev"]
         assert s["info"]["its"] == ds["its"]
+
+
+@pytest.mark.parametrize("dims,bndtype", [((20, 20, 20), 1), ((30, 30, 1), 2)])
+def test_lan_trbounds_against_oracle(lib, dims, bndtype):
+    nx, ny, nz = dims
+    banner("LanTrbounds %dx%dx%d bndtype %d vs oracle" % (nx, ny, nz, bndtype))
+    ia, ja, a = driver.laplacian3d(nx, ny, nz)
+    M = driver.Matrix(lib, ia, ja, a)
+    inst = driver.Instance(lib, M)  # pEVSL_Start seeds srand(0)
+    try:
+        v = driver.rand_vinit(lib, M.n)
+        rc, lo, hi = inst.lan_trbounds(50, 200, 1e-8, v, bndtype)
+    finally:
+        inst.close()
+        M.free()
+    assert rc == 0
+    orc.srand(0)
+    ov = orc.rand_vec(nx * ny * nz)
+    assert np.array_equal(ov, v)
+    orc_rc, olo, ohi, its = orc.lan_trbounds(orc.Csr(ia, ja, a), 50, 200, 1e-8, ov, bndtype)
+    assert orc_rc == 0
+    # same algorithm and start vector; reduction orders differ, so the
+    # converged bounds agree to the stopping tolerance
+    scale = abs(olo) + abs(ohi)
+    assert abs(lo - olo) < 1e-8 * scale and abs(hi - ohi) < 1e-8 * scale
+    ev = driver.exact_eig_lap3(nx, ny, nz, -1.0, 13.0)
+    assert abs(lo - ev[0]) < 1e-6 * scale and abs(hi - ev[-1]) < 1e-6 * scale
+
+
+def test_lan_trbounds_unconverged_state_matches_oracle(lib):
+    banner("LanTrbounds 60^3, 200 products (not converged) vs oracle")
+    ia, ja, a = driver.laplacian3d(60, 60, 60)
+    M = driver.Matrix(lib, ia, ja, a)
+    inst = driver.Instance(lib, M)
+    try:
+        v = driver.rand_vinit(lib, M.n)
+        rc, lo, hi = inst.lan_trbounds(50, 200, 1e-8, v, 1)
+    finally:
+        inst.close()
+        M.free()
+    orc.srand(0)
+    ov = orc.rand_vec(M.n)
+    orc_rc, olo, ohi, its = orc.lan_trbounds(orc.Csr(ia, ja, a), 50, 200, 1e-8, ov, 1)
+    assert rc == orc_rc == 0 and its == 200
+    assert abs(lo - olo) < 1e-9 * 12 and abs(hi - ohi) < 1e-9 * 12
+
+
+def test_lan_trbounds_100cube_closed_form(lib):
+    banner("LanTrbounds 100^3 vs closed-form extremes")
+    ia, ja, a = driver.laplacian3d(100, 100, 100)
+    M = driver.Matrix(lib, ia, ja, a)
+    inst = driver.Instance(lib, M)
+    try:
+        # the ends of the 100^3 spectrum are 3e-3 apart: about 10^3 products
+        # to reach the 1e-8 stopping rule with a 50-vector restart
+        rc, lo, hi = inst.lan_trbounds(50, 5000, 1e-8)
+    finally:
+        inst.close()
+        M.free()
+    assert rc == 0
+    t = np.sin(np.pi / (2 * 101.0)) ** 2 * 12
+    tmax = np.sin(100 * np.pi / (2 * 101.0)) ** 2 * 12
+    assert abs(lo - t) < 1e-6 * 12 and abs(hi - tmax) < 1e-6 * 12
+    assert lo <= t + 1e-12 and hi >= tmax - 1e-12
