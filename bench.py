@@ -209,7 +209,8 @@ def main():
                  "found": mfound, "exact": sum(r["exact"] for r in mres),
                  "max_rel_err": max((r["max_rel_err"] or 0.0) for r in mres),
                  "max_residual": max((r["max_res"] or 0.0) for r in mres),
-                 "tolerance": "residual < 1e-5 (FP32-filtered basis), eigenvalue rel err < 1e-10",
+                 "tolerance": "residual < 1e-6 * max(|lmin|, |lmax|) = 1.2e-5 (FP32-filtered basis), "
+                              "eigenvalue rel err < 1e-10",
                  "phase_s": {"filter": round(ksm["t_filter"], 3), "cgs": round(ksm["t_cgs"], 3),
                              "eig": round(ksm["t_eig"], 3), "ritz": round(ksm["t_ritz"], 3)},
                  "filter_GBs": round(ksm["filter_bytes"] / max(ksm["t_filter"], 1e-9) / 1e9, 1),

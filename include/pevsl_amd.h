@@ -63,11 +63,13 @@ int pEVSL_SetFilterPrecision(pevsl_Data *pevsl, int prec);
  * reference. */
 int pEVSL_SetBasisPrecision(pevsl_Data *pevsl, int prec);
 
-/* Residual threshold used instead of ChebLanNr's tol when the filter runs in
- * FP32 (default 1e-5): Ritz vectors built from an FP32-filtered Krylov basis
- * carry FP32-level errors (~1e-6 relative), while their FP64 Rayleigh
- * quotients stay accurate to ~1e-12.  The accepted set and the reported
- * residuals use this threshold; eigenvalue errors are reported separately. */
+/* Relative residual threshold of the FP32-filter path (default 1e-6): a
+ * Ritz pair is accepted when ||A u - theta u|| < max(tol, mixed_tol * rho),
+ * rho = max(|lmin|, |lmax|) of the spectrum frame (1.2e-5 for the 3-D
+ * Laplacian's [0, 12]).  Ritz vectors built from an FP32-filtered Krylov
+ * basis carry residuals of about 1e-7 rho, while their FP64 Rayleigh-Ritz
+ * values stay accurate to ~1e-12 relative; eigenvalue errors are reported
+ * separately. */
 int pEVSL_SetMixedTol(pevsl_Data *pevsl, double tol);
 
 /* Ritz stage: 0 = the reference's per-vector Ritz pairs (SRC/cheblanNr.c:

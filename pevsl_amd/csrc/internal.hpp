@@ -76,7 +76,7 @@ struct Ctx {
   void *host_mv_data = nullptr;
   int filter_prec = PEVSL_PREC_FP64;
   int basis_prec = PEVSL_PREC_FP64;  // Lanczos basis storage (FP32 only with an FP32 filter)
-  double mixed_tol = 1e-5;         // residual threshold of the FP32-filter path
+  double mixed_tol = 1e-6;         // FP32-filter path: residual threshold / spectral radius
   int ritz_refine = -1;            // FP64 Rayleigh-Ritz refinement: 1 on, 0 off, -1 FP32 only
   int rand_mode = PEVSL_RAND_LIBC;
   unsigned long long rand_seed = 0;

@@ -231,6 +231,14 @@ class BatchSolver {
   // FP64 view of basis chunk ci of slice s (kc columns): the chunk itself, or
   // its widened copy in conv_ when the basis is stored in FP32
   const double *chunk64(int s, int ci, int kc);
+  // residual threshold of slice s: tol, or on the FP32-filter path
+  // max(tol, mixed_tol * rho) with rho = max(|lmin|, |lmax|) of the frame
+  double accept_tol(int s, double tol) const {
+    if (c_.filter_prec != PEVSL_PREC_FP32) return tol;
+    const pevsl_Polparams *pol = S_[(size_t)s].pol;
+    const double rho = std::max(std::fabs(pol->cc - pol->dd), std::fabs(pol->cc + pol->dd));
+    return std::max(tol, c_.mixed_tol * rho);
+  }
   // z = p(A) x between basis columns / an FP64 input, in the basis' type
   void filter_into(int s, const double *x64, int kin, int kout);
   void ensure_column(int s, int j);
@@ -649,8 +657,7 @@ void BatchSolver::ritz(int s, double tol) {
       rr((size_t)CH);
   const int csz = 1 << L_.chunk_shift;
   const double one = 1.0, zero = 0.0;
-  const double acc_tol =
-      c_.filter_prec == PEVSL_PREC_FP32 ? std::max(tol, c_.mixed_tol) : tol;
+  const double acc_tol = accept_tol(s, tol);
   for (int cs = 0; cs < nsel && sl.rc == 0; cs += CH) {
     const int nb = std::min(CH, nsel - cs);
     // U = V(:, 0:kdim) * C(:, i0+cs : i0+cs+nb), accumulated over basis chunks
@@ -820,8 +827,7 @@ void BatchSolver::ritz_rr(int s, double tol) {
   col_dots(n_, nsel, U.p, ld, U.p, ld, part.p, red.p, st);
   PV_HIP(hipMemcpyAsync(rr.data(), red.p, sizeof(double) * nsel, hipMemcpyDeviceToHost, st));
   PV_HIP(hipStreamSynchronize(st));
-  const double acc_tol =
-      c_.filter_prec == PEVSL_PREC_FP32 ? std::max(tol, c_.mixed_tol) : tol;
+  const double acc_tol = accept_tol(s, tol);
   for (int i = 0; i < nsel; i++) {
     const double t = tq[(size_t)i];
     if (t < sl.aa - DBL_EPSILON || t > sl.bb + DBL_EPSILON) continue;

@@ -48,6 +48,43 @@ def laplacian3d(nx, ny, nz):
     return ia, ja, a
 
 
+def powerlaw_sym(n, mean=16.0, max_len=2000, alpha=2.2, diag_frac=0.1, seed=0):
+    """Seeded irregular test matrix: row i of A0 has L_i entries, L_i drawn
+    from a power law P(L) ~ L^-alpha truncated at max_len and scaled to the
+    given mean, at uniformly random columns with values U(-1, 1);
+    A = A0 + A0^T (duplicates summed) plus diag(diag_frac * sum_j |A_ij|).
+    Returns (ia, ja, a) as int32/int32/float64 CSR with sorted columns."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+
+    def tmean(xmin):  # mean of the continuous power law on [xmin, max_len]
+        r = xmin / max_len
+        return xmin * (alpha - 1) / (alpha - 2) * (1 - r ** (alpha - 2)) / (1 - r ** (alpha - 1))
+    lo, hi = 0.5, mean
+    for _ in range(100):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if tmean(mid) < mean else (lo, mid)
+    xmin = 0.5 * (lo + hi)
+    u = rng.random(n)
+    r = xmin / max_len
+    # inverse CDF of the truncated power law
+    x = xmin * (1 - u * (1 - r ** (alpha - 1))) ** (-1.0 / (alpha - 1))
+    L = np.clip(np.rint(x), 1, max_len).astype(np.int64)
+    rows = np.repeat(np.arange(n, dtype=np.int64), L)
+    cols = rng.integers(0, n, size=rows.size, dtype=np.int64)
+    vals = rng.uniform(-1.0, 1.0, size=rows.size)
+    A0 = sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr()
+    del rows, cols, vals
+    A = (A0 + A0.T).tocsr()
+    del A0
+    A.sum_duplicates()
+    d = diag_frac * np.asarray(abs(A).sum(axis=1)).ravel()
+    A = (A + sp.diags(d)).tocsr()
+    A.sort_indices()
+    return (A.indptr.astype(np.int32), A.indices.astype(np.int32),
+            A.data.astype(np.float64))
+
+
 def exact_eig_lap3(nx, ny, nz, a, b):
     """Closed-form eigenvalues in [a - eps, b + eps], sorted
     (TESTS/COMMON/lapl.c:139-169)."""
@@ -278,6 +315,84 @@ def cheblan_batch(inst, intvs, maxits, tol, vinit, pols, keep_vectors=False,
             libc.free(C.cast(reso[s], C.c_void_p))
         out.append({"lam": lam, "res": res, "vecs": vecs, "info": inst.lan_info(s)})
     return rc, out
+
+
+def irregular_slicing_run(n, nslices=16, seed=4, frac=0.05, center=0.5, nvec=128, Mdeg=300,
+                          prec=capi.PEVSL_PREC_FP64, basis_prec=capi.PEVSL_PREC_FP64,
+                          batch=True, wave=None, slices_only=None, log=None, verbose=False,
+                          tune=True):
+    """The irregular-sparsity workload: powerlaw_sym(n) (mean row length 16
+    before symmetrization, max 2000, exponent 2.2), bounds by
+    pEVSL_LanTrbounds (lanm 50, maxit 200, tol 1e-8, as TESTS/MM/MMPLanN.c:111),
+    interval = a `frac` share of [lmin, lmax] centred at the `center` share
+    (0.5: the middle of the spectrum), KPM (Mdeg, nvec) slicing
+    into `nslices`, FindPol and (batched) ChebLanNr as in lap_slicing_run.
+    Returns the DOS, slices and per-slice results (no closed form: callers
+    check counts against their own reference)."""
+    lib = capi.load()
+    t0 = time.time()
+    ia, ja, av = powerlaw_sym(n, seed=seed)
+    M = Matrix(lib, ia, ja, av)
+    out = {"n": n, "nnz": M.nnz, "t_gen": time.time() - t0}
+    inst = Instance(lib, M, prec=prec, basis_prec=basis_prec)
+    try:
+        if tune:
+            tr = capi.TuneResult()
+            lib.pEVSL_Autotune(inst.p, 10, C.byref(tr))
+            out["autotune"] = {"lanes": tr.lanes, "block_width": tr.block_width,
+                               "block_kernel": tr.block_rowkernel}
+        t1 = time.time()
+        rc, lmin, lmax = inst.lan_trbounds(50, 200, 1e-8)
+        if rc:
+            raise RuntimeError("LanTrbounds error %d" % rc)
+        c, h = lmin + center * (lmax - lmin), 0.5 * frac * (lmax - lmin)
+        a, b = c - h, c + h
+        xintv = np.array([a, b, lmin, lmax])
+        rc, mu, ecount = inst.kpmdos(Mdeg, 1, nvec, xintv)
+        if rc:
+            raise RuntimeError("kpmdos error %d" % rc)
+        rc, sli = spslicer_kpm(lib, mu, xintv, nslices, max(int(10 * ecount), 2 * nslices + 1))
+        if rc:
+            raise RuntimeError("spslicer error %d" % rc)
+        out.update(bounds=[lmin, lmax], intv=[a, b], ecount=ecount, sli=sli, mu=mu,
+                   t_dos=time.time() - t1)
+        ev_int = int(1 + ecount / float(nslices))
+        vinit = rand_vinit(lib, n)
+        nev = ev_int + 2
+        mlan = min(max(5 * nev, 300), n)
+        todo = list(range(nslices)) if slices_only is None else list(slices_only)
+        pols = [find_pol(lib, [sli[s], sli[s + 1], lmin, lmax])[0] for s in todo]
+        intvs = [[sli[s], sli[s + 1], lmin, lmax] for s in todo]
+        if batch:
+            w_ = wave or len(todo)
+            groups = [list(range(i, min(i + w_, len(todo)))) for i in range(0, len(todo), w_)]
+        else:
+            groups = [[i] for i in range(len(todo))]
+        t3 = time.time()
+        res = []
+        for g in groups:
+            rc, outs = cheblan_batch(inst, [intvs[i] for i in g], [mlan] * len(g), 1e-8, vinit,
+                                     [pols[i] for i in g], False, verbose)
+            if rc:
+                raise RuntimeError("ChebLanNr error %d" % rc)
+            for i, o in zip(g, outs):
+                res.append({"slice": todo[i], "intv": intvs[i][:2], "deg": pols[i].deg,
+                            "nev": len(o["lam"]), "lam": np.sort(o["lam"]), "res": o["res"],
+                            "info": o["info"], "mlan": mlan})
+                if log:
+                    log("slice %d: deg %d its %s nev %d" % (
+                        todo[i], pols[i].deg, o["info"]["its"] if o["info"] else "?",
+                        len(o["lam"])))
+        out["t_lanczos"] = time.time() - t3
+        out["slices"] = res
+        out["kstats"] = inst.kernel_stats()
+        for p_ in pols:
+            lib.pEVSL_FreePol(C.byref(p_))
+    finally:
+        inst.close()
+        M.free()
+    out["t_total"] = time.time() - t0
+    return out
 
 
 def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
