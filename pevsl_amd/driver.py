@@ -85,6 +85,97 @@ def powerlaw_sym(n, mean=16.0, max_len=2000, alpha=2.2, diag_frac=0.1, seed=0):
             A.data.astype(np.float64))
 
 
+def read_matrix_market(path, idxin=1):
+    """Sparse real/integer coordinate Matrix Market file -> (ia, ja, a) CSR
+    (int32/int32/float64, columns sorted), with the reference reader's rules
+    (TESTS/COMMON/parmm.c:11-162): indices start at idxin (1 for MM1, 0 for
+    MM0); a 'symmetric' file stores one triangle and every off-diagonal
+    entry is mirrored.  Duplicate entries are summed."""
+    with open(path, "r") as f:
+        banner = f.readline().split()
+        if len(banner) < 5 or banner[0] != "%%MatrixMarket" or banner[1].lower() != "matrix":
+            raise ValueError("%s: not a Matrix Market matrix" % path)
+        fmt, field, symm = banner[2].lower(), banner[3].lower(), banner[4].lower()
+        if fmt != "coordinate" or field not in ("real", "integer"):
+            raise ValueError("%s: only sparse real/integer coordinate matrices" % path)
+        line = f.readline()
+        while line.startswith("%"):
+            line = f.readline()
+        nrow, ncol, nnz = (int(v) for v in line.split()[:3])
+        data = np.loadtxt(f, dtype=np.float64, ndmin=2, max_rows=nnz)
+    if data.shape[0] != nnz:
+        raise ValueError("%s: %d entries, header says %d" % (path, data.shape[0], nnz))
+    r = data[:, 0].astype(np.int64) - idxin
+    c = data[:, 1].astype(np.int64) - idxin
+    v = data[:, 2]
+    if symm == "symmetric":
+        off = r != c
+        r, c, v = np.concatenate([r, c[off]]), np.concatenate([c, r[off]]), np.concatenate([v, v[off]])
+    import scipy.sparse as sp
+    A = sp.coo_matrix((v, (r, c)), shape=(nrow, ncol)).tocsr()
+    A.sum_duplicates()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+
+
+def mm_slicing_run(path, a, b, nslices, idxin=1, prec=capi.PEVSL_PREC_FP64,
+                   basis_prec=capi.PEVSL_PREC_FP64, batch=False, log=None, verbose=False):
+    """End-to-end run of the TESTS/MM/MMPLanN.c flow (KPM branch): Matrix
+    Market matrix, bounds by pEVSL_LanTrbounds(50, 200, 1e-8), KPM DOS
+    (Mdeg 50, nvec 60, Jackson) and SpslicerKpm with npts = 10 ecount,
+    FindPol (damping 2, thresholds 0.6/0.15, max_deg 3000), ChebLanNr with
+    mlan = min(max(5 nev, 300), n), tol 1e-8, one vinit for all slices."""
+    lib = capi.load()
+    t0 = time.time()
+    ia, ja, av = read_matrix_market(path, idxin)
+    M = Matrix(lib, ia, ja, av)
+    n = M.n
+    out = {"n": n, "nnz": M.nnz}
+    inst = Instance(lib, M, prec=prec, basis_prec=basis_prec)
+    try:
+        vinit = rand_vinit(lib, n)
+        rc, lmin, lmax = inst.lan_trbounds(50, 200, 1e-8, vinit, 1)
+        if rc:
+            raise RuntimeError("LanTrbounds error %d" % rc)
+        xintv = np.array([a, b, lmin, lmax])
+        rc, mu, ecount = inst.kpmdos(50, 1, 60, xintv)
+        if rc:
+            raise RuntimeError("kpmdos error %d" % rc)
+        rc, sli = spslicer_kpm(lib, mu, xintv, nslices, int(10 * ecount))
+        if rc:
+            raise RuntimeError("spslicer error %d" % rc)
+        out.update(bounds=[lmin, lmax], ecount=ecount, sli=sli, t_dos=time.time() - t0)
+        ev_int = int(1 + ecount / float(nslices))
+        nev = ev_int + 2
+        mlan = min(max(5 * nev, 300), n)
+        pols = [find_pol(lib, [sli[s], sli[s + 1], lmin, lmax], damping=2, thresh_int=0.6,
+                         thresh_ext=0.15, max_deg=3000)[0] for s in range(nslices)]
+        intvs = [[sli[s], sli[s + 1], lmin, lmax] for s in range(nslices)]
+        groups = [list(range(nslices))] if batch else [[s] for s in range(nslices)]
+        t3 = time.time()
+        res = []
+        for g in groups:
+            rc, outs = cheblan_batch(inst, [intvs[i] for i in g], [mlan] * len(g), 1e-8, vinit,
+                                     [pols[i] for i in g], False, verbose)
+            if rc:
+                raise RuntimeError("ChebLanNr error %d" % rc)
+            for i, o in zip(g, outs):
+                res.append({"slice": i, "intv": intvs[i][:2], "deg": pols[i].deg,
+                            "nev": len(o["lam"]), "lam": np.sort(o["lam"]), "res": o["res"],
+                            "info": o["info"], "mlan": mlan})
+                if log:
+                    log("slice %d: deg %d nev %d" % (i, pols[i].deg, len(o["lam"])))
+        out["t_lanczos"] = time.time() - t3
+        out["slices"] = res
+        for p_ in pols:
+            lib.pEVSL_FreePol(C.byref(p_))
+    finally:
+        inst.close()
+        M.free()
+    out["t_total"] = time.time() - t0
+    return out
+
+
 def exact_eig_lap3(nx, ny, nz, a, b):
     """Closed-form eigenvalues in [a - eps, b + eps], sorted
     (TESTS/COMMON/lapl.c:139-169)."""

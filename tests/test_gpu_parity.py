@@ -488,3 +488,29 @@ def test_irregular_small_counts_match_dense(lib, prec, basis):
             assert np.all(np.asarray(s["res"]) < tol_res)
         total += s["nev"]
     assert total == np.sum((ev >= r["intv"][0]) & (ev <= r["intv"][1])) > 100
+
+
+@pytest.mark.parametrize("a,b,ns", [(0.10, 0.12, 3), (0.12, 0.14, 2)])
+def test_matrix_market_flow_stiff1(lib, a, b, ns):
+    """The TESTS/MM/MMPLanN.c flow on the reference's stiff1 matrix with the
+    intervals of its TESTS/MM/matfile: every slice finds the dense-eigensolve
+    count, eigenvalues to 1e-10 relative."""
+    import json
+    import os
+    banner("MM flow stiff1 [%g, %g] %d slices" % (a, b, ns))
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    g = json.load(open(os.path.join(gdir, "mm_stiff1_eigs.json")))
+    r = driver.mm_slicing_run(os.path.join(gdir, "mm_stiff1.mtx"), a, b, ns, idxin=1)
+    lmin, lmax = r["bounds"]
+    assert lmin <= g["lambda_min"] + 1e-8 and lmax >= g["lambda_max"] - 1e-8
+    ev = np.array(g["eigs_0.10_0.14"])
+    total = 0
+    for s in r["slices"]:
+        lo, hi = s["intv"]
+        ex = ev[(ev >= lo) & (ev <= hi)]
+        assert s["nev"] == len(ex), (s["slice"], s["nev"], len(ex))
+        if len(ex):
+            assert np.max(np.abs(s["lam"] - ex) / np.abs(ex)) < 1e-10
+            assert np.all(np.asarray(s["res"]) < 1e-8)
+        total += s["nev"]
+    assert total == np.sum((ev >= a) & (ev <= b))
