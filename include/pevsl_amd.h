@@ -92,6 +92,14 @@ int pEVSL_SetRandMode(pevsl_Data *pevsl, int mode, unsigned long long seed);
 /* Whether ChebLanNr copies the Ritz vectors back (*Wo); 1 by default. */
 int pEVSL_SetKeepVectors(pevsl_Data *pevsl, int keep);
 
+/* Seeds the library's rand() stream.  Random vectors in PEVSL_RAND_LIBC
+ * mode (pEVSL_ParvecRand, the KPM and Lanczos-DOS vectors) follow glibc's
+ * rand() algorithm after srand(seed), in a generator private to the library,
+ * so other users of libc rand() in the process do not shift them.
+ * pEVSL_Start seeds it with 0, as the reference's srand(rank) on one rank
+ * (SRC/pevsl.c:36). */
+void pEVSL_RandSeed(unsigned seed);
+
 /* Kernel of the one-column filter step: lanes per row of the vector-CSR
  * kernel (1..64, power of two), PEVSL_SPMV_STREAM for the CSR-stream kernel
  * (row blocks staged in LDS), or 0 = automatic; and the block width of
@@ -104,8 +112,11 @@ int pEVSL_SetSpmvTuning(pevsl_Data *pevsl, int lanes_per_row, int block_width);
 /* Block SpMM kernel of batched solves: 1 = one thread per row with B-wide
  * row gathers (B <= 32) and the reference's three-term recurrence with an
  * accumulator (5 vector passes per degree), 2 = the same kernel in Clenshaw
- * form (4 passes), 0 = one lane per (row, column).  Default (-1): 2 for an
- * FP32 filter, 1 for FP64. */
+ * form (4 passes), 3 / 4 = the nnz-balanced forms of 1 / 2 (products formed
+ * in parallel over nonzeros and staged in LDS, long rows reduced by a whole
+ * workgroup: for uneven row lengths), 0 = one lane per (row, column).
+ * Default (-1): 2 for an FP32 filter, 1 for FP64, or 4 / 3 when the
+ * matrix has a row longer than 64 entries. */
 int pEVSL_SetBlockKernel(pevsl_Data *pevsl, int rowkernel);
 
 /* Eigenvalues >= thr of the symmetric tridiagonal matrix (diag[0..n-1],
@@ -164,10 +175,11 @@ typedef struct {
   double bytes_block[6];
   double t_stream;    /* CSR-stream kernel, seconds per step */
   double t_blockrow[6]; /* row-per-thread block kernel, widths 2..64 (0: n/a) */
-  int block_rowkernel;  /* chosen block kernel: 2 row-per-thread Clenshaw,
-                           1 row-per-thread, 0 lane per (row, column) */
+  int block_rowkernel;  /* chosen block kernel (pEVSL_SetBlockKernel codes) */
   double t_blockclen[6];  /* Clenshaw row kernel, widths 2..64 (FP32 only) */
   double bytes_clen[6];   /* its algorithmic bytes per step */
+  double t_blockbal[6];   /* nnz-balanced three-term kernel, widths 2..64 */
+  double t_blockbalclen[6];  /* nnz-balanced Clenshaw kernel (FP32 only) */
 } pevsl_TuneResult;
 int pEVSL_Autotune(pevsl_Data *pevsl, int reps, pevsl_TuneResult *res);
 

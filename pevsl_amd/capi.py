@@ -94,7 +94,8 @@ class TuneResult(C.Structure):
                 ("t_block", C.c_double * 6), ("bytes_block", C.c_double * 6),
                 ("t_stream", C.c_double), ("t_blockrow", C.c_double * 6),
                 ("block_rowkernel", C.c_int), ("t_blockclen", C.c_double * 6),
-                ("bytes_clen", C.c_double * 6)]
+                ("bytes_clen", C.c_double * 6), ("t_blockbal", C.c_double * 6),
+                ("t_blockbalclen", C.c_double * 6)]
 
 
 class KernelStats(C.Structure):
@@ -224,6 +225,7 @@ SIGNATURES = {
     "pEVSL_SetRandMode": (C.c_int, [DataP, C.c_int, C.c_ulonglong]),
     "pEVSL_SetKeepVectors": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetBlockKernel": (C.c_int, [DataP, C.c_int]),
+    "pEVSL_RandSeed": (None, [C.c_uint]),
     "pEVSL_SetSpmvTuning": (C.c_int, [DataP, C.c_int, C.c_int]),
     "pEVSL_ChebLanNrBatch": (C.c_int, [DataP, C.c_int, c_double_p, c_int_p,
                                        C.c_double, C.POINTER(Parvec),

@@ -71,12 +71,52 @@ void apply_A(Ctx &c, const double *dx, double *dy) {
                         hipMemcpyHostToDevice, c.stream));
 }
 
+// The reference draws its random vectors from libc rand() after srand(rank)
+// (SRC/pevsl.c:36, SRC/parvec.c:65-72).  The library keeps that stream in a
+// generator of its own -- glibc's rand() algorithm (additive feedback,
+// r_i = r_{i-31} + r_{i-3} mod 2^32 over a state seeded by the multiplier-16807
+// congruential generator, first 310 outputs discarded, output r_i >> 1) -- so
+// that the values equal the reference's and no other user of libc rand() in
+// the process (runtime or math libraries initialising themselves) can shift
+// them.
+namespace {
+struct RandStream {
+  uint32_t r[34];  // r[k mod 34] = r_k for the last 34 k
+  int i = 4;       // next k mod 34
+  void seed(unsigned s) {
+    int32_t w = s ? (int32_t)s : 1;
+    uint32_t full[344];
+    full[0] = (uint32_t)w;
+    for (int k = 1; k < 31; k++) {
+      // 16807 * w mod (2^31 - 1) by Schrage's method
+      const int32_t hi = w / 127773, lo = w % 127773;
+      w = 16807 * lo - 2836 * hi;
+      if (w < 0) w += 2147483647;
+      full[k] = (uint32_t)w;
+    }
+    for (int k = 31; k < 34; k++) full[k] = full[k - 31];
+    for (int k = 34; k < 344; k++) full[k] = full[k - 31] + full[k - 3];
+    for (int k = 310; k < 344; k++) r[k % 34] = full[k];
+    i = 344 % 34;
+  }
+  int next() {
+    // r_k = r_{k-31} + r_{k-3}: slots (k + 3) and (k + 31) mod 34
+    const uint32_t v = r[(i + 3) % 34] + r[(i + 31) % 34];
+    r[i] = v;
+    i = (i + 1) % 34;
+    return (int)(v >> 1);
+  }
+};
+RandStream g_rand;
+int lib_rand() { return g_rand.next(); }
+}  // namespace
+
 void rand_vectors(Ctx &c, int n, int nvec, double *d_out, size_t ld) {
   if (c.rand_mode == PEVSL_RAND_LIBC) {
     std::vector<double> h((size_t)n);
     const double t = ((double)RAND_MAX) / 2.0;
     for (int v = 0; v < nvec; v++) {
-      for (int i = 0; i < n; i++) h[i] = (rand() - t) / t;
+      for (int i = 0; i < n; i++) h[i] = (lib_rand() - t) / t;
       PV_HIP(hipMemcpyAsync(d_out + (size_t)v * ld, h.data(),
                             sizeof(double) * n, hipMemcpyHostToDevice,
                             c.stream));
@@ -123,8 +163,8 @@ int pEVSL_Start(MPI_Comm comm, pevsl_Data **data) {
   p->n = PEVSL_NOT_DEFINED;
   p->nfirst = PEVSL_NOT_DEFINED;
   p->sigma_mult = 1.0;
-  // one process: rank 0 seeds libc rand() (SRC/pevsl.c:331-336)
-  srand(0);
+  // one process: rank 0 seeds the rand() stream (SRC/pevsl.c:331-336)
+  g_rand.seed(0);
   PEVSL_MALLOC(p->stats, 1, pevsl_Stat);
   Ctx *c = new Ctx();
   PV_HIP(hipGetDevice(&c->device));
@@ -265,7 +305,7 @@ int pEVSL_SetSpmvTuning(pevsl_Data *p, int lanes, int bw) {
 }
 
 int pEVSL_SetBlockKernel(pevsl_Data *p, int rowkernel) {
-  if (rowkernel < -1 || rowkernel > 2) return -1;
+  if (rowkernel < -1 || rowkernel > 4) return -1;
   ctx_of(p).blk_rowkernel = rowkernel;
   return 0;
 }
@@ -362,6 +402,15 @@ int pEVSL_ParcsrSetup(pevsl_Csr *Ai, pevsl_Parcsr *A) {
   M->csr.nrowblk = (int)rb.size() - 1;
   M->csr.rowblk = M->rowblk.ensure(rb.size());
   PV_HIP(hipMemcpy(M->csr.rowblk, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice));
+  M->csr.rowblk_q[0] = M->csr.rowblk;
+  M->csr.nrowblk_q[0] = M->csr.nrowblk;
+  for (int l = 1; l < 5; l++) {
+    build_row_blocks(n, Ai->ia, rb, kStreamRows >> l, kStreamNnz >> l);
+    M->csr.nrowblk_q[l] = (int)rb.size() - 1;
+    M->csr.rowblk_q[l] = M->rowblk_q[l].ensure(rb.size());
+    PV_HIP(hipMemcpy(M->csr.rowblk_q[l], rb.data(), sizeof(int) * rb.size(),
+                     hipMemcpyHostToDevice));
+  }
   std::vector<float> a32((size_t)nnz);
   for (int j = 0; j < nnz; j++) a32[j] = (float)Ai->a[j];
   PV_HIP(hipMemcpy(M->csr.ia, Ai->ia, sizeof(int) * (size_t)(n + 1),
@@ -479,8 +528,10 @@ void pEVSL_ParvecFree(pevsl_Parvec *x) { PEVSL_FREE(x->data); }
 
 void pEVSL_ParvecRand(pevsl_Parvec *x) {
   const double t = ((double)RAND_MAX) / 2.0;
-  for (int i = 0; i < x->n_local; i++) x->data[i] = (rand() - t) / t;
+  for (int i = 0; i < x->n_local; i++) x->data[i] = (lib_rand() - t) / t;
 }
+
+void pEVSL_RandSeed(unsigned seed) { g_rand.seed(seed); }
 
 void pEVSL_ParvecDot(pevsl_Parvec *x, pevsl_Parvec *y, double *t) {
   double s = 0.0;

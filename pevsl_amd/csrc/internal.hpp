@@ -62,9 +62,14 @@ struct DBuf {
 struct DevMatrix {
   DevCsr csr;
   DBuf<int> ia, ja, rowblk;
+  DBuf<int> rowblk_q[5];  // partitions of the nnz-balanced kernels, Q >= 2
   DBuf<double> a64;
   DBuf<float> a32;
 };
+
+// Rows longer than this make the batched filter default to the nnz-balanced
+// block kernels.
+constexpr int kUnevenRow = 64;
 
 // Per-instance device context (attached to pevsl_Data.dev).
 struct Ctx {

@@ -440,6 +440,238 @@ __global__ void k_blk_colscale(size_t n, int B, const T *__restrict__ v, T *__re
   out[g] = (T)mus[(size_t)k * B + (int)(g % B)] * v[g];
 }
 
+// ---- nnz-balanced block products -------------------------------------------
+// Block SpMM for B interleaved columns on matrices whose row lengths are very
+// uneven (power-law graphs), where a thread that walks a whole row stalls its
+// wave for the longest row.  Per row block:
+//   phase 1: one thread per (nonzero, 16-byte piece of the B-wide row of x):
+//            every product a_ij * x_j(piece) is formed in parallel, with
+//            independent gathers, and staged in LDS;
+//   phase 2: one thread per (row, piece) sums the row's staged products in
+//            column order and runs the fused epilogue on its piece.
+// A row with more nonzeros than a block stages is a block alone and is reduced
+// by the whole workgroup (shuffles, then LDS across waves).  Products are
+// rounded before they are added and staged rows are summed in column order,
+// so those rows round exactly as in the row-per-thread kernels.
+//
+// Budget: NZ = kStreamNnz / Q staged products of B values (32 KiB of LDS when
+// a piece is 16 bytes) and at most kStreamRows / Q rows per block, Q = pieces
+// per row; the host builds one row partition per Q (DevCsr::rowblk_q).
+template <typename T, int B>
+struct Pieces {
+  static constexpr int P = (B * (int)sizeof(T) >= 16) ? 16 / (int)sizeof(T) : B;
+  static constexpr int Q = B / P;
+  static constexpr int NZ = kStreamNnz / Q;
+};
+
+template <typename T, int P>
+__device__ __forceinline__ void ld_piece(const T *p, T (&x)[P]) {
+  if constexpr (P * sizeof(T) == 16) {
+    *reinterpret_cast<float4 *>(x) = *reinterpret_cast<const float4 *>(p);
+  } else if constexpr (P * sizeof(T) == 8) {
+    *reinterpret_cast<float2 *>(x) = *reinterpret_cast<const float2 *>(p);
+  } else {
+#pragma unroll
+    for (int e = 0; e < P; e++) x[e] = p[e];
+  }
+}
+template <typename T, int P>
+__device__ __forceinline__ void st_piece(T *p, const T (&x)[P]) {
+  if constexpr (P * sizeof(T) == 16) {
+    *reinterpret_cast<float4 *>(p) = *reinterpret_cast<const float4 *>(x);
+  } else if constexpr (P * sizeof(T) == 8) {
+    *reinterpret_cast<float2 *>(p) = *reinterpret_cast<const float2 *>(x);
+  } else {
+#pragma unroll
+    for (int e = 0; e < P; e++) p[e] = x[e];
+  }
+}
+
+// Epilogue of the three-term step (same operations as k_cheb_blkrow) on
+// columns [piece*P, piece*P + P) of row r.
+template <typename T, int B>
+struct EpiCheb {
+  const T *vk, *vkm1;
+  T *vkp1, *y;
+  T cc, t;
+  const double *mus;
+  int k, nact;
+  __device__ __forceinline__ void operator()(int r, int piece,
+                                             const T (&acc)[Pieces<T, B>::P]) const {
+#pragma clang fp contract(off)
+    constexpr int P = Pieces<T, B>::P;
+    const size_t o = (size_t)r * B + piece * P;
+    T v[P], yy[P], w[P];
+    ld_piece<T, P>(vk + o, v);
+    ld_piece<T, P>(y + o, yy);
+    if (k == 1) {
+#pragma unroll
+      for (int e = 0; e < P; e++) {
+        const int c = piece * P + e;
+        w[e] = acc[e] + (-cc) * v[e];
+        w[e] = w[e] * t;
+        if (c < nact) yy[e] = v[e] * (T)mus[c] + (T)mus[(size_t)B + c] * w[e];
+      }
+    } else {
+      T vm[P];
+      ld_piece<T, P>(vkm1 + o, vm);
+#pragma unroll
+      for (int e = 0; e < P; e++) {
+        const int c = piece * P + e;
+        w[e] = acc[e] + (-cc) * v[e];
+        w[e] = w[e] * t;
+        w[e] = w[e] + (T)(-1) * vm[e];
+        if (c < nact) yy[e] = yy[e] + (T)mus[(size_t)k * B + c] * w[e];
+      }
+    }
+    st_piece<T, P>(vkp1 + o, w);
+    st_piece<T, P>(y + o, yy);
+  }
+};
+
+// Epilogue of the Clenshaw step (same operations as k_clen_blkrow); b1 is the
+// gathered block.
+template <typename T, int B>
+struct EpiClen {
+  const T *b1, *b2, *v;
+  T *out;
+  T cc, t;
+  const double *mus;
+  int k;
+  __device__ __forceinline__ void operator()(int r, int piece,
+                                             const T (&acc)[Pieces<T, B>::P]) const {
+#pragma clang fp contract(off)
+    constexpr int P = Pieces<T, B>::P;
+    const size_t o = (size_t)r * B + piece * P;
+    T bb[P], vv[P], b2r[P], w[P];
+    ld_piece<T, P>(b1 + o, bb);
+    ld_piece<T, P>(v + o, vv);
+    if (b2) {
+      ld_piece<T, P>(b2 + o, b2r);
+    } else {
+#pragma unroll
+      for (int e = 0; e < P; e++) b2r[e] = 0;
+    }
+#pragma unroll
+    for (int e = 0; e < P; e++) {
+      T z = acc[e] + (-cc) * bb[e];
+      z = z * t;
+      z = z + (T)mus[(size_t)k * B + piece * P + e] * vv[e];
+      w[e] = z + (T)(-1) * b2r[e];
+    }
+    st_piece<T, P>(out + o, w);
+  }
+};
+
+// U = NZ / S = 8 products per thread and block: each thread first issues
+// its 8 index and value loads (nontemporal: the matrix streams once per step
+// and should not evict the gathered block from the caches), then its 8
+// gathers, so that 8 gathers per thread are in flight at once.
+constexpr int kBalUnroll = 8;
+
+template <typename T, int B, class Epi>
+__global__ void __launch_bounds__(kThreads)
+k_blk_bal(const int *__restrict__ rowblk, const int *__restrict__ ia,
+          const int *__restrict__ ja, const T *__restrict__ a, const T *__restrict__ x,
+          Epi epi) {
+#pragma clang fp contract(off)
+  using PC = Pieces<T, B>;
+  constexpr int P = PC::P, Q = PC::Q, S = kThreads / Q, U = kBalUnroll;
+  static_assert(PC::NZ == U * S, "one unrolled batch covers a staged block");
+  __shared__ T prod[PC::NZ * B];
+  const int rb = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = rowblk[rb], r1 = rowblk[rb + 1];
+  const int j0 = ia[r0], nz = ia[r1] - j0;
+  const int piece = threadIdx.x % Q, slot = threadIdx.x / Q;
+  // products q = q0 + slot + i*S, i < U, of the nonzeros j0 + q < j0 + nz
+  auto batch = [&](int q0, T (&xv)[U][P], T (&av)[U]) {
+    int col[U];
+#pragma unroll
+    for (int i = 0; i < U; i++) {
+      const int q = q0 + slot + i * S;
+      col[i] = 0;
+      av[i] = 0;
+      if (q < nz) {
+        col[i] = __builtin_nontemporal_load(ja + j0 + q);
+        av[i] = __builtin_nontemporal_load(a + j0 + q);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < U; i++) {
+      if (q0 + slot + i * S < nz) ld_piece<T, P>(x + (size_t)col[i] * B + piece * P, xv[i]);
+    }
+  };
+  if (nz > PC::NZ) {
+    // one long row (the host partition never puts it with others)
+    T acc[P];
+#pragma unroll
+    for (int e = 0; e < P; e++) acc[e] = 0;
+    for (int q0 = 0; q0 < nz; q0 += U * S) {
+      T xv[U][P], av[U];
+      batch(q0, xv, av);
+#pragma unroll
+      for (int i = 0; i < U; i++) {
+        if (q0 + slot + i * S < nz) {
+#pragma unroll
+          for (int e = 0; e < P; e++) acc[e] += av[i] * xv[i][e];
+        }
+      }
+    }
+    // lanes of a wave that hold the same piece: lane % Q == piece
+#pragma unroll
+    for (int off = 32; off >= Q; off >>= 1) {
+#pragma unroll
+      for (int e = 0; e < P; e++) acc[e] += __shfl_xor(acc[e], off, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane < Q) {
+#pragma unroll
+      for (int e = 0; e < P; e++) prod[wave * B + piece * P + e] = acc[e];
+    }
+    __syncthreads();
+    if (threadIdx.x < Q) {
+#pragma unroll
+      for (int e = 0; e < P; e++) {
+        T sum = 0;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; w++) sum += prod[w * B + piece * P + e];
+        acc[e] = sum;
+      }
+      epi(r0, piece, acc);
+    }
+    return;
+  }
+  {
+    T xv[U][P], av[U];
+    batch(0, xv, av);
+#pragma unroll
+    for (int i = 0; i < U; i++) {
+      const int q = slot + i * S;
+      if (q < nz) {
+#pragma unroll
+        for (int e = 0; e < P; e++) xv[i][e] = av[i] * xv[i][e];
+        st_piece<T, P>(prod + (size_t)q * B + piece * P, xv[i]);
+      }
+    }
+  }
+  __syncthreads();
+  const int nrq = (r1 - r0) * Q;
+  for (int u = threadIdx.x; u < nrq; u += kThreads) {
+    const int r = r0 + u / Q, pc = u % Q;
+    const int qs = ia[r] - j0, qe = ia[r + 1] - j0;
+    T acc[P];
+#pragma unroll
+    for (int e = 0; e < P; e++) acc[e] = 0;
+    for (int q = qs; q < qe; q++) {
+      T pv[P];
+      ld_piece<T, P>(prod + (size_t)q * B + pc * P, pv);
+#pragma unroll
+      for (int e = 0; e < P; e++) acc[e] += pv[e];
+    }
+    epi(r, pc, acc);
+  }
+}
+
 // KPM step + per-column partial dot with v0 over the block's rows.
 template <typename T, int B>
 __global__ void __launch_bounds__(kThreads)
@@ -1181,20 +1413,87 @@ void cheb_step_vec(const DevCsr &A, const T *vk, const T *vkm1, T *vkp1, T *y,
                          (T)muk, first ? 1 : 0);
   }
 }
-void build_row_blocks(int n, const int *ia, std::vector<int> &blk) {
+void build_row_blocks(int n, const int *ia, std::vector<int> &blk, int max_rows, int max_nnz) {
   blk.clear();
   blk.push_back(0);
   int r = 0;
   while (r < n) {
     const int r0 = r;
-    if (ia[r + 1] - ia[r] > kStreamNnz) {
+    if (ia[r + 1] - ia[r] > max_nnz) {
       r++;
     } else {
-      while (r < n && r - r0 < kStreamRows && ia[r + 1] - ia[r0] <= kStreamNnz) r++;
+      while (r < n && r - r0 < max_rows && ia[r + 1] - ia[r0] <= max_nnz) r++;
     }
     blk.push_back(r);
   }
 }
+
+int bal_level(int elem, int B) {
+  int q = B * elem / 16, l = 0;
+  while ((1 << l) < q) l++;
+  return l;
+}
+
+template <typename T>
+void cheb_step_bal(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
+                   T *y, double cc, double t, const double *mus, int k,
+                   int nact, hipStream_t s) {
+  if (A.n == 0 || nact <= 0) return;
+  const int lv = bal_level((int)sizeof(T), B);
+  if (lv > 4 || !A.rowblk_q[lv] || A.nrowblk_q[lv] == 0) return;
+  const T *a = std::is_same<T, double>::value ? (const T *)A.a64 : (const T *)A.a32;
+  switch (B) {
+#define CASE(BB)                                                                         \
+  case BB: {                                                                             \
+    const EpiCheb<T, BB> epi{vk, vkm1, vkp1, y, (T)cc, (T)t, mus, k, nact};              \
+    hipLaunchKernelGGL((k_blk_bal<T, BB, EpiCheb<T, BB>>), dim3(A.nrowblk_q[lv]),        \
+                       dim3(kThreads), 0, s, A.rowblk_q[lv], A.ia, A.ja, a, vk, epi);    \
+    break;                                                                               \
+  }
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32)
+#undef CASE
+    default:
+      break;
+  }
+}
+template <typename T>
+void clen_step_bal(const DevCsr &A, int B, const T *b1, const T *b2, const T *v, T *out,
+                   double cc, double t, const double *mus, int k, hipStream_t s) {
+  if (A.n == 0) return;
+  if (!b1) {  // top term: no product
+    hipLaunchKernelGGL((k_blk_colscale<T>), dim3(grid_for((size_t)A.n * B)), dim3(kThreads), 0,
+                       s, (size_t)A.n, B, v, out, mus, k);
+    return;
+  }
+  const int lv = bal_level((int)sizeof(T), B);
+  if (lv > 4 || !A.rowblk_q[lv] || A.nrowblk_q[lv] == 0) return;
+  const T *a = std::is_same<T, double>::value ? (const T *)A.a64 : (const T *)A.a32;
+  switch (B) {
+#define CASE(BB)                                                                         \
+  case BB: {                                                                             \
+    const EpiClen<T, BB> epi{b1, b2, v, out, (T)cc, (T)t, mus, k};                       \
+    hipLaunchKernelGGL((k_blk_bal<T, BB, EpiClen<T, BB>>), dim3(A.nrowblk_q[lv]),        \
+                       dim3(kThreads), 0, s, A.rowblk_q[lv], A.ia, A.ja, a, b1, epi);    \
+    break;                                                                               \
+  }
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32)
+#undef CASE
+    default:
+      break;
+  }
+}
+template void cheb_step_bal<double>(const DevCsr &, int, const double *, const double *,
+                                    double *, double *, double, double, const double *, int,
+                                    int, hipStream_t);
+template void cheb_step_bal<float>(const DevCsr &, int, const float *, const float *, float *,
+                                   float *, double, double, const double *, int, int,
+                                   hipStream_t);
+template void clen_step_bal<double>(const DevCsr &, int, const double *, const double *,
+                                    const double *, double *, double, double, const double *,
+                                    int, hipStream_t);
+template void clen_step_bal<float>(const DevCsr &, int, const float *, const float *,
+                                   const float *, float *, double, double, const double *, int,
+                                   hipStream_t);
 
 template <typename T>
 void cheb_step_stream(const DevCsr &A, const T *vk, const T *vkm1, T *vkp1,

@@ -22,14 +22,21 @@ struct DevCsr {
   // kStreamRows rows and kStreamNnz nonzeros (a longer row is a block alone)
   int *rowblk = nullptr;
   int nrowblk = 0;
+  // row blocks of the nnz-balanced block kernels, one partition per number
+  // Q = 2^i of 16-byte pieces in a B-wide row: at most kStreamRows/Q rows and
+  // kStreamNnz/Q nonzeros per block (rowblk_q[0] == rowblk)
+  int *rowblk_q[5] = {};
+  int nrowblk_q[5] = {};
 };
 
 constexpr int kStreamRows = 256;
 constexpr int kStreamNnz = 2048;
 
-// host: row-block partition for the CSR-stream kernels (returns nrowblk+1
-// offsets)
-void build_row_blocks(int n, const int *ia, std::vector<int> &blk);
+// host: row-block partition with at most max_rows rows and max_nnz nonzeros
+// per block, a longer row being a block alone (returns nrowblk+1 offsets);
+// the CSR-stream kernels use (kStreamRows, kStreamNnz)
+void build_row_blocks(int n, const int *ia, std::vector<int> &blk,
+                      int max_rows = kStreamRows, int max_nnz = kStreamNnz);
 
 // ---- sparse products -------------------------------------------------------
 // y = A x, one column.  lanes = lanes per row in {1,2,4,8,16,32,64}.
@@ -75,6 +82,19 @@ void cheb_step_blkrow(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp
 template <typename T>
 void clen_step_blkrow(const DevCsr &A, int B, const T *b1, const T *b2, const T *v, T *out,
                       double cc, double t, const double *mus, int k, hipStream_t s);
+
+// nnz-balanced forms of cheb_step_blkrow and clen_step_blkrow (same
+// arguments and results; products formed in parallel over nonzeros and
+// staged in LDS, long rows reduced by a whole workgroup); B <= 32.
+template <typename T>
+void cheb_step_bal(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
+                   T *y, double cc, double t, const double *mus, int k,
+                   int nact, hipStream_t s);
+template <typename T>
+void clen_step_bal(const DevCsr &A, int B, const T *b1, const T *b2, const T *v, T *out,
+                   double cc, double t, const double *mus, int k, hipStream_t s);
+// partition index of the balanced kernels for element size elem, width B
+int bal_level(int elem, int B);
 
 // KPM step on B interleaved columns:
 //   vkp1 = scal (A vk - ctr vk) - vkm1,   part[blk][c] = sum_rows vkp1*v0

@@ -399,11 +399,16 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
   const double cc = S_[(size_t)grp[0]].pol->cc, dd = S_[(size_t)grp[0]].pol->dd;
   std::vector<int> degs;
   for (int s : grp) degs.push_back(S_[(size_t)s].pol->deg);
-  // row-per-thread kernel (default) up to 32 columns; its Clenshaw form is
-  // the default on the FP32 path (the FP64 path keeps the reference's
-  // three-term recurrence and accumulation order)
-  const bool rowk = c_.blk_rowkernel != 0 && B <= 32 && A.nrowblk > 0;
-  const bool clen = rowk && (c_.blk_rowkernel == 2 || (c_.blk_rowkernel < 0 && f32));
+  // block kernel: 1 row per thread, 2 its Clenshaw form, 3 / 4 the
+  // nnz-balanced forms of 1 / 2, 0 lane per (row, column).  Default: the
+  // Clenshaw form on the FP32 path, the reference's three-term recurrence on
+  // FP64, and the balanced forms when some row is long.
+  int kind = c_.blk_rowkernel;
+  if (kind < 0) kind = (f32 ? 2 : 1) + (A.max_row > kUnevenRow ? 2 : 0);
+  if (B > 32 || A.nrowblk == 0) kind = 0;
+  const bool rowk = kind != 0;
+  const bool clen = kind == 2 || kind == 4;
+  const bool bal = kind >= 3;
   auto run = [&](auto *tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
     T *x = (T *)xb[0], *y = (T *)xb[1];
@@ -434,8 +439,12 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
           out = ring[nr];
         }
         nr = (nr + 1) % 3;
-        clen_step_blkrow<T>(A, B, b1, b2, x, out, cc, d == 0 ? 1.0 / dd : 2.0 / dd, mus_.p, d,
-                            c_.stream);
+        if (bal)
+          clen_step_bal<T>(A, B, b1, b2, x, out, cc, d == 0 ? 1.0 / dd : 2.0 / dd, mus_.p, d,
+                           c_.stream);
+        else
+          clen_step_blkrow<T>(A, B, b1, b2, x, out, cc, d == 0 ? 1.0 / dd : 2.0 / dd, mus_.p,
+                              d, c_.stream);
         int nact = 0;  // columns whose series has reached degree d + 1
         while (nact < g && degs[(size_t)nact] >= d + 1) nact++;
         if (b1) {
@@ -462,7 +471,10 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
         vkp1 = bufs[nb];
       }
       nb = (nb + 1) % 3;
-      if (rowk)
+      if (bal)
+        cheb_step_bal<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd, mus_.p, d,
+                         nact, c_.stream);
+      else if (rowk)
         cheb_step_blkrow<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd,
                             mus_.p, d, nact, c_.stream);
       else

@@ -48,7 +48,7 @@ double time_vec(Ctx &c, int lanes, int reps) {
 }
 
 // kind: 0 lane per (row, column), 1 row per thread, 2 row per thread in
-// Clenshaw form
+// Clenshaw form, 3 / 4 the nnz-balanced forms of 1 / 2
 template <typename T>
 double time_blk(Ctx &c, int B, int reps, DBuf<double> &mus, int kind) {
   const DevCsr &A = *c.A;
@@ -58,7 +58,11 @@ double time_blk(Ctx &c, int B, int reps, DBuf<double> &mus, int kind) {
   PV_HIP(hipMemsetAsync(b.p, 0, sizeof(T) * 4 * blk, c.stream));
   T *v0 = b.p, *v1 = b.p + blk, *v2 = b.p + 2 * blk, *y = b.p + 3 * blk;
   auto step = [&](const T *vk, const T *vkm1, T *vkp1) {
-    if (kind == 2)
+    if (kind == 4)
+      clen_step_bal<T>(A, B, vk, vkm1, y, vkp1, 6.0, 1.0 / 3, mus.p, 2, c.stream);
+    else if (kind == 3)
+      cheb_step_bal<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
+    else if (kind == 2)
       clen_step_blkrow<T>(A, B, vk, vkm1, y, vkp1, 6.0, 1.0 / 3, mus.p, 2, c.stream);
     else if (kind == 1)
       cheb_step_blkrow<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
@@ -133,7 +137,15 @@ int pEVSL_Autotune(pevsl_Data *p, int reps, pevsl_TuneResult *res) {
         bw = B;
         rowk = 1;
       }
-      // Clenshaw form: FP32 only (the FP64 path keeps the reference's
+      const double tb = f32 ? time_blk<float>(c, B, reps, mus, 3)
+                            : time_blk<double>(c, B, reps, mus, 3);
+      res->t_blockbal[i] = tb;
+      if (tb / B < bestc) {
+        bestc = tb / B;
+        bw = B;
+        rowk = 3;
+      }
+      // Clenshaw forms: FP32 only (the FP64 path keeps the reference's
       // recurrence)
       if (f32) {
         const double tc = time_blk<float>(c, B, reps, mus, 2);
@@ -143,6 +155,13 @@ int pEVSL_Autotune(pevsl_Data *p, int reps, pevsl_TuneResult *res) {
           bestc = tc / B;
           bw = B;
           rowk = 2;
+        }
+        const double tcb = time_blk<float>(c, B, reps, mus, 4);
+        res->t_blockbalclen[i] = tcb;
+        if (tcb / B < bestc) {
+          bestc = tcb / B;
+          bw = B;
+          rowk = 4;
         }
       }
     }

@@ -411,21 +411,22 @@ def cheblan_batch(inst, intvs, maxits, tol, vinit, pols, keep_vectors=False,
 def irregular_slicing_run(n, nslices=16, seed=4, frac=0.05, center=0.5, nvec=128, Mdeg=300,
                           prec=capi.PEVSL_PREC_FP64, basis_prec=capi.PEVSL_PREC_FP64,
                           batch=True, wave=None, slices_only=None, log=None, verbose=False,
-                          tune=True):
+                          tune=True, block_rowkernel=-1):
     """The irregular-sparsity workload: powerlaw_sym(n) (mean row length 16
     before symmetrization, max 2000, exponent 2.2), bounds by
     pEVSL_LanTrbounds (lanm 50, maxit 200, tol 1e-8, as TESTS/MM/MMPLanN.c:111),
     interval = a `frac` share of [lmin, lmax] centred at the `center` share
     (0.5: the middle of the spectrum), KPM (Mdeg, nvec) slicing
     into `nslices`, FindPol and (batched) ChebLanNr as in lap_slicing_run.
-    Returns the DOS, slices and per-slice results (no closed form: callers
-    check counts against their own reference)."""
+    block_rowkernel selects the block SpMM (pEVSL_SetBlockKernel; an
+    autotune run overrides it).  Returns the DOS, slices and per-slice results
+    (no closed form: callers check counts against their own reference)."""
     lib = capi.load()
     t0 = time.time()
     ia, ja, av = powerlaw_sym(n, seed=seed)
     M = Matrix(lib, ia, ja, av)
     out = {"n": n, "nnz": M.nnz, "t_gen": time.time() - t0}
-    inst = Instance(lib, M, prec=prec, basis_prec=basis_prec)
+    inst = Instance(lib, M, prec=prec, basis_prec=basis_prec, block_rowkernel=block_rowkernel)
     try:
         if tune:
             tr = capi.TuneResult()

@@ -195,3 +195,29 @@ def test_generalized_setters_report_not_implemented(lib):
     # LanTrbounds is implemented (standard problem); missing arguments are an
     # error, not a crash
     assert lib.pEVSL_LanTrbounds(None, 10, 10, 1e-8, None, 0, None, None, None) == -1
+
+
+@pytest.mark.parametrize("seed", [0, 1, 7, 123456789])
+def test_rand_stream_equals_libc_rand(lib, seed):
+    """pEVSL_ParvecRand draws from the library's own copy of the rand()
+    stream; after pEVSL_RandSeed(s) it equals libc srand(s)/rand() (the
+    reference's SRC/parvec.c:65-72 values), here checked against the C
+    library itself over 2000 draws (past the 344-word warm-up state)."""
+    libc = C.CDLL(None)
+    libc.srand.argtypes = [C.c_uint]
+    libc.rand.restype = C.c_int
+    libc.srand(seed)
+    t = 2147483647 / 2.0
+    want = (np.array([libc.rand() for _ in range(2000)], dtype=np.float64) - t) / t
+    n = 2000
+    v = np.empty(n)
+    pv = capi.Parvec()
+    lib.pEVSL_RandSeed(seed)
+    lib.pEVSL_ParvecCreateShell(n, n, 0, 0, C.byref(pv), v.ctypes.data_as(capi.c_double_p))
+    lib.pEVSL_ParvecRand(C.byref(pv))
+    assert np.array_equal(v, want)
+    # a draw from libc in between does not shift the library's stream
+    lib.pEVSL_RandSeed(seed)
+    libc.rand()
+    lib.pEVSL_ParvecRand(C.byref(pv))
+    assert np.array_equal(v, want)

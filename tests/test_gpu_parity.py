@@ -200,7 +200,7 @@ def test_c1_readme_slices(lib):
         assert s["nev"] == gs["found"] == gs["exact"]
 
 
-@pytest.mark.parametrize("rowk", [0, 1])
+@pytest.mark.parametrize("rowk", [0, 1, 3])
 def test_batch_equals_single_slices(lib, rowk):
     banner("batched vs one-by-one solve, 16^3, 4 slices, block kernel rowk=%d" % rowk)
     r1 = driver.lap_slicing_run(16, 16, 16, 0.4, 1.0, 4, batch=False)
@@ -324,13 +324,14 @@ def test_tridiag_multisection_against_lapack(lib, n):
     assert abs(tr.value - sel.sum()) <= 64 * np.finfo(float).eps * scale * np.sqrt(n) * max(1, len(sel))
 
 
-@pytest.mark.parametrize("batch,rowk", [(False, -1), (True, 1), (True, 2)])
+@pytest.mark.parametrize("batch,rowk", [(False, -1), (True, 1), (True, 2), (True, 3), (True, 4)])
 def test_mixed_precision_c1(lib, batch, rowk):
     banner("C1 mixed precision (FP32 filter + FP64 Rayleigh-Ritz) batch=%s block kernel %d"
            % (batch, rowk))
     # stated mixed-path tolerance: residual < 1e-6 * rho (rho = 12 here),
     # eigenvalue error < 1e-10;
-    # block kernel 1 = three-term recurrence, 2 = Clenshaw form
+    # block kernel 1 = three-term recurrence, 2 = Clenshaw form, 3 / 4 their
+    # nnz-balanced forms
     r = driver.lap_slicing_run(20, 20, 20, 0.6, 1.2, 5, batch=batch, prec=capi.PEVSL_PREC_FP32,
                                block_rowkernel=rowk)
     for s in r["slices"]:
@@ -458,17 +459,24 @@ def test_lan_trbounds_100cube_closed_form(lib):
     assert lo <= t + 1e-12 and hi >= tmax - 1e-12
 
 
-@pytest.mark.parametrize("prec,basis", [(capi.PEVSL_PREC_FP64, capi.PEVSL_PREC_FP64),
-                                        (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32)])
-def test_irregular_small_counts_match_dense(lib, prec, basis):
+@pytest.mark.parametrize("prec,basis,rowk", [
+    (capi.PEVSL_PREC_FP64, capi.PEVSL_PREC_FP64, -1),
+    (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, -1),
+    (capi.PEVSL_PREC_FP64, capi.PEVSL_PREC_FP64, 1),
+    (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, 2)])
+def test_irregular_small_counts_match_dense(lib, prec, basis, rowk):
     """The irregular-sparsity recipe (power-law rows, Lanczos bounds, a 5%
     share of the spectrum, 16 KPM slices) at n = 3000, where the dense
     eigenvalues give every slice's exact count.  At this size the middle of
     the spectrum is empty, so the interval is centred at 20% of [lmin, lmax]
-    (184 eigenvalues)."""
-    banner("irregular n=3000, 16 slices, prec %d basis %d, batched" % (prec, basis))
+    (184 eigenvalues).  Rows reach ~2000 entries, so the default (-1) runs
+    the nnz-balanced block kernels with their whole-workgroup long-row path
+    (16 columns: rows over 256 FP64 / 512 FP32 entries); 1 and 2 are the
+    row-per-thread kernels."""
+    banner("irregular n=3000, 16 slices, prec %d basis %d, block kernel %d, batched"
+           % (prec, basis, rowk))
     r = driver.irregular_slicing_run(3000, nslices=16, seed=11, nvec=64, center=0.2, prec=prec,
-                                     basis_prec=basis, tune=False)
+                                     basis_prec=basis, tune=False, block_rowkernel=rowk)
     ia, ja, a = driver.powerlaw_sym(3000, seed=11)
     import scipy.sparse as sp
     ev = np.linalg.eigvalsh(sp.csr_matrix((a, ja, ia), shape=(3000, 3000)).toarray())
