@@ -57,18 +57,35 @@ void pEVSL_SpslicerLan(double *xi, double *yi, int n_int, int npts,
                        double *sli);
 
 /* ---- lantrbnd.c (INC/pevsl.h:28-30) --------------------------------------
- * Spectrum bounds by thick-restart Lanczos (standard problem; the
- * generalized one returns PEVSL_NOT_IMPLEMENT).        SRC/lantrbnd.c:31 */
+ * Spectrum bounds by thick-restart Lanczos, standard or generalized
+ * (B-inner products, after pEVSL_SetGenEig).            SRC/lantrbnd.c:31 */
 int pEVSL_LanTrbounds(pevsl_Data *pevsl, int lanm, int maxit, double tol,
                       pevsl_Parvec *vinit, int bndtype, double *lammin,
                       double *lammax, FILE *fstats);
 
+/* ---- lspolapprox.c (INC/pevsl.h:31-37) -----------------------------------
+ * Least-squares Chebyshev approximation of B^{-1/2} on [lmin, lmax] (degree
+ * <= max_deg, relative error sum < tol) for the L^T solve of a generalized
+ * problem, its solve (host arrays; installed with pEVSL_SetBSol /
+ * pEVSL_SetLTSol it runs on the device with no host copies) and its
+ * release.  The fit is host work; solves need B set up with
+ * pEVSL_ParcsrSetup (its HBM copy).
+ *                                       SRC/lspolapprox.c:63-146, 190-322 */
+void pEVSL_SetupLSPolSqrt(int max_deg, double tol, double lmin, double lmax,
+                          pevsl_Parcsr *B, void **vdata);
+void pEVSL_LSPolSol(double *db, double *dx, void *data);
+void pEVSL_LSPolFree(void *vdata);
+
 /* ---- pevsl.c (INC/pevsl.h:40-51) -----------------------------------------
- * Start seeds libc rand() with the rank (0) as SRC/pevsl.c:36 does, creates
- * the HIP stream and device context.  SetAParcsr copies the CSR to HBM and
- * installs the device matvec (SRC/pevsl.c:81-97).  SetAMatvec installs a
- * host callback (every product then crosses PCIe).  The generalized-problem
- * setters (B, B-solve, L^T-solve) return PEVSL_NOT_IMPLEMENT. */
+ * Start seeds the library's rand() stream (glibc's algorithm) with the rank
+ * (0) as SRC/pevsl.c:36 seeds libc rand(), and creates the HIP stream and
+ * device context.  SetAParcsr / SetBParcsr use the matrix's HBM copy
+ * (SRC/pevsl.c:81-121).  SetAMatvec / SetBMatvec install host callbacks
+ * (every product then crosses PCIe).  SetBSol / SetLTSol install the solves
+ * of a generalized problem (host callbacks, or pEVSL_LSPolSol, which runs on
+ * the device); SetGenEig switches Kpmdos, LanTrbounds, ChebAv and ChebLanNr
+ * to A x = lambda B x (SRC/pevsl.c:123-210).  The generalized LanDosG
+ * returns PEVSL_NOT_IMPLEMENT. */
 int pEVSL_Start(MPI_Comm comm, pevsl_Data **data);
 int pEVSL_Finish(pevsl_Data *pevsl_data);
 int pEVSL_SetProbSizes(pevsl_Data *pevsl_data, int N, int n, int nfirst);

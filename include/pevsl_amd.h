@@ -92,6 +92,17 @@ int pEVSL_SetRandMode(pevsl_Data *pevsl, int mode, unsigned long long seed);
 /* Whether ChebLanNr copies the Ritz vectors back (*Wo); 1 by default. */
 int pEVSL_SetKeepVectors(pevsl_Data *pevsl, int keep);
 
+/* Least-squares Chebyshev approximations for generalized problems
+ * (SRC/lspolapprox.c:63-190): of B^{-1} (SetupLSPolRec, a B-solve), and of
+ * any positive function ffun of B (SetupLSPol).  pEVSL_LSPolInfo returns the
+ * chosen degree, the deg + 1 coefficients (mu may be NULL) and the interval
+ * map x -> (x - cc)/dd.  Free with pEVSL_LSPolFree. */
+void pEVSL_SetupLSPolRec(int max_deg, double tol, double lmin, double lmax,
+                         pevsl_Parcsr *B, void **vdata);
+void pEVSL_SetupLSPol(int max_deg, double tol, double lmin, double lmax,
+                      pevsl_Parcsr *B, double (*ffun)(double), void **vdata);
+int pEVSL_LSPolInfo(void *vdata, int *deg, double *mu, double *cc, double *dd);
+
 /* Seeds the library's rand() stream.  Random vectors in PEVSL_RAND_LIBC
  * mode (pEVSL_ParvecRand, the KPM and Lanczos-DOS vectors) follow glibc's
  * rand() algorithm after srand(seed), in a generator private to the library,

@@ -1,0 +1,102 @@
+"""Dense numpy restatement (FP64) of the generalized-problem pieces of the
+path: the least-squares Chebyshev fit of the B-solves, the generalized
+Chebyshev filter and the generalized KPM moments.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker.  The product
+(pevsl_amd) never imports it.
+
+How each function is pinned (tests/test_gen_oracle.py):
+  lspol_coeffs   closed-form Chebyshev series of 1/(cc + dd t); a degree-200
+                 Chebyshev interpolant of t -> (cc + dd t)^(-1/2); the
+                 degree rule recomputed with numpy.polynomial.chebyshev
+  cheb_av_gen    the spectral form p(A B^{-1}) = B X p(Lambda) X^T of the
+                 pencil (X B-orthonormal eigenvectors, scipy.linalg.eigh)
+  kpm_gen_dots   the spectral form w^T B X T_k(Lambda) X^T B w
+"""
+import math
+
+import numpy as np
+
+
+def lspol_coeffs(f, a, b, max_deg, tol):
+    """Chebyshev coefficients of f on [a, b] by Gauss-Chebyshev quadrature
+    on 4 max_deg nodes; the degree grows until the sum of the relative
+    errors at 50 max_deg equispaced points of [-1, 1] is below tol
+    (SRC/lspolapprox.c:63-146).  Returns (mu[0..deg], cc, dd)."""
+    cc, dd = (a + b) / 2.0, (b - a) / 2.0
+    npts = 4 * max_deg
+    th = (2 * np.arange(npts) + 1) * (math.pi / (2 * npts))
+    g = np.array([f(cc + dd * math.cos(t)) for t in th])
+    m = 50 * max_deg
+    h = 2.0 / (m - 1) if m > 1 else 0.0  # linspace of SRC/vector.c:15-25
+    xs = np.array([-1.0 + i * h for i in range(m)])
+    xs[0], xs[-1] = -1.0, 1.0
+    fx = np.array([f(cc + dd * x) for x in xs])
+    mu = np.zeros(max(max_deg, 1))
+    k = 0
+    while k < max_deg:
+        mu[k] = (1.0 if k == 0 else 2.0) / npts * float(np.sum(np.cos(k * th) * g))
+        err = float(np.sum(np.abs((cheb_series(mu[:k + 1], xs) - fx) / fx)))
+        if err < tol:
+            k += 1
+            break
+        k += 1
+    deg = k - 1
+    return mu[:deg + 1].copy(), cc, dd
+
+
+def cheb_series(mu, x):
+    """sum_j mu_j T_j(x) by the three-term recurrence (chebxPltd,
+    SRC/chebpol.c:100-131)."""
+    x = np.asarray(x, dtype=np.float64)
+    tkm1, tk = np.zeros_like(x), np.ones_like(x)
+    y = mu[0] * np.ones_like(x)
+    for k in range(len(mu) - 1):
+        tkp1 = (1.0 if k == 0 else 2.0) * x * tk - tkm1
+        tkm1, tk = tk, tkp1
+        y = y + mu[k + 1] * tk
+    return y
+
+
+def lspol_apply(B, mu, cc, dd, b):
+    """sum_k mu_k T_k((B - cc)/dd) b (pnav, SRC/lspolapprox.c:222-268)."""
+    vk, vkm1 = b.copy(), np.zeros_like(b)
+    y = mu[0] * vk
+    for k in range(1, len(mu)):
+        t = (1.0 if k == 1 else 2.0) / dd
+        vkp1 = (B @ vk - cc * vk) * t - (vkm1 if k > 1 else 0.0)
+        y = y + mu[k] * vkp1
+        vkm1, vk = vk, vkp1
+    return y
+
+
+def cheb_av_gen(A, solveB, mu, cc, dd, v):
+    """Generalized ChebAv (SRC/chebpol.c:509-574 with ifGenEv): per degree
+    w2 = B \\ v_k, v_{k+1} = t (A w2 - cc v_k) - v_{k-1}, y += mu_k v_{k+1}."""
+    vk, vkm1 = v.copy(), np.zeros_like(v)
+    y = mu[0] * vk
+    for k in range(1, len(mu)):
+        t = (1.0 if k == 1 else 2.0) / dd
+        vkp1 = (A @ solveB(vk) - cc * vk) * t - (vkm1 if k > 1 else 0.0)
+        y = y + mu[k] * vkp1
+        vkm1, vk = vk, vkp1
+    return y
+
+
+def kpm_gen_dots(A, B, solveB, solveLT, vecs, Mdeg, ctr, wid):
+    """dots[m, k] of the generalized KPM (SRC/kpmdos.c:105-156): for each
+    random vector (rows of vecs), v = unit(v), w = L^{-T} v, v = B w,
+    v_0 = w, v_{k+1} = scal (B^{-1} A v_k - ctr v_k) - v_{k-1},
+    dots[m, k] = <v_{k+1}, v>."""
+    out = np.zeros((len(vecs), Mdeg))
+    for m, v in enumerate(vecs):
+        v = v / np.linalg.norm(v)
+        w = solveLT(v)
+        v = B @ w
+        vk, vkm1 = w.copy(), np.zeros_like(w)
+        for k in range(Mdeg):
+            scal = (1.0 if k == 0 else 2.0) / wid
+            vkp1 = (solveB(A @ vk) - ctr * vk) * scal - vkm1
+            vkm1, vk = vk, vkp1
+            out[m, k] = vk @ v
+    return out

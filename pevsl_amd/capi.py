@@ -226,6 +226,16 @@ SIGNATURES = {
     "pEVSL_SetKeepVectors": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetBlockKernel": (C.c_int, [DataP, C.c_int]),
     "pEVSL_RandSeed": (None, [C.c_uint]),
+    "pEVSL_SetupLSPolSqrt": (None, [C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.POINTER(Parcsr), C.POINTER(C.c_void_p)]),
+    "pEVSL_SetupLSPolRec": (None, [C.c_int, C.c_double, C.c_double, C.c_double,
+                                   C.POINTER(Parcsr), C.POINTER(C.c_void_p)]),
+    "pEVSL_SetupLSPol": (None, [C.c_int, C.c_double, C.c_double, C.c_double,
+                                C.POINTER(Parcsr), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "pEVSL_LSPolSol": (None, [c_double_p, c_double_p, C.c_void_p]),
+    "pEVSL_LSPolFree": (None, [C.c_void_p]),
+    "pEVSL_LSPolInfo": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), c_double_p, c_double_p,
+                                  c_double_p]),
     "pEVSL_SetSpmvTuning": (C.c_int, [DataP, C.c_int, C.c_int]),
     "pEVSL_ChebLanNrBatch": (C.c_int, [DataP, C.c_int, c_double_p, c_int_p,
                                        C.c_double, C.POINTER(Parvec),

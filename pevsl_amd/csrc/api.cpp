@@ -233,15 +233,10 @@ int pEVSL_SetAMatvec(pevsl_Data *p, MVFunc func, void *data) {
   return 0;
 }
 
-int pEVSL_SetBParcsr(pevsl_Data *, pevsl_Parcsr *) { return PEVSL_NOT_IMPLEMENT; }
-int pEVSL_SetBMatvec(pevsl_Data *, MVFunc, void *) { return PEVSL_NOT_IMPLEMENT; }
-int pEVSL_SetBSol(pevsl_Data *, SVFunc, void *) { return PEVSL_NOT_IMPLEMENT; }
-int pEVSL_SetLTSol(pevsl_Data *, SVFunc, void *) { return PEVSL_NOT_IMPLEMENT; }
 int pEVSL_SetStdEig(pevsl_Data *p) {
   p->ifGenEv = 0;
   return 0;
 }
-int pEVSL_SetGenEig(pevsl_Data *) { return PEVSL_NOT_IMPLEMENT; }
 int pEVSL_SetSigmaMult(pevsl_Data *p, double mult) {
   p->sigma_mult = mult;
   return 0;

@@ -150,7 +150,7 @@ int pEVSL_Kpmdos(pevsl_Data *pevsl, int Mdeg, int damping, int nvec, double *int
   Ctx &c = ctx_of(pevsl);
   if (ngroups > 1) return PEVSL_NOT_IMPLEMENT;  // one process: one group
   if (check_intv(intv, stdout) < 0) return -1;
-  if (!c.A) return PEVSL_NOT_IMPLEMENT;  // KPM needs the device matrix
+  if (!c.A && !pevsl->ifGenEv) return PEVSL_NOT_IMPLEMENT;  // block KPM needs the device matrix
   const int n = pevsl->n;
   const double aa = std::max(intv[0], intv[2]), bb = std::min(intv[1], intv[3]);
   if (intv[0] < intv[2] || intv[1] > intv[3])
@@ -167,7 +167,9 @@ int pEVSL_Kpmdos(pevsl_Data *pevsl, int Mdeg, int damping, int nvec, double *int
   std::vector<double> dots((size_t)nvec * Mdeg);
   EvTimer ev;
   ev.start(c.stream);
-  if (c.filter_prec == PEVSL_PREC_FP32)
+  if (pevsl->ifGenEv)
+    kpm_moments_gen(c, pevsl, Mdeg, nvec, ctr, wid, dots);
+  else if (c.filter_prec == PEVSL_PREC_FP32)
     kpm_moments<float>(c, n, Mdeg, nvec, ctr, wid, dots);
   else
     kpm_moments<double>(c, n, Mdeg, nvec, ctr, wid, dots);
@@ -308,6 +310,9 @@ int pEVSL_LanDosG(pevsl_Data *pevsl, const int nvec, int msteps, int npts, doubl
   (void)groupid;
   (void)gl_comm;
   if (ngroups > 1) return PEVSL_NOT_IMPLEMENT;
+  // the generalized Lanczos DOS (SRC/landos.c ifGenEv branches) is not built:
+  // generalized problems slice with pEVSL_Kpmdos
+  if (pevsl->ifGenEv) return PEVSL_NOT_IMPLEMENT;
   Ctx &c = ctx_of(pevsl);
   const double tm = wtime();
   const int n = pevsl->n;

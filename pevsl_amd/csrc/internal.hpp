@@ -99,6 +99,10 @@ struct Ctx {
   std::vector<double> h_x, h_y;
   // scratch
   DBuf<double> scratch64;
+  // generalized problem: device B (SetBParcsr) and work space
+  const DevCsr *B = nullptr;
+  std::vector<double> hb_x, hb_y;
+  DBuf<double> gen_lsw, gen_dot, gen_cgs, gen_fw, gen_tri;
 };
 
 Ctx &ctx_of(pevsl_Data *p);
@@ -145,5 +149,27 @@ void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
                      double *dy);
 // The same for FP32 columns (FP32 filter arithmetic; device matrix only).
 void cheb_filter_col32(Ctx &c, const pevsl_Polparams *pol, const float *dv, float *dy);
+
+// ---- generalized problem (gen.cpp) -----------------------------------------
+// y = B x, x = B \ b, x = L^{-T} b on device vectors through the installed
+// operator / solve (device CSR, the library's LSPol solve on the device, or a
+// host callback with host copies).
+void apply_B(Ctx &c, pevsl_Data *p, const double *dx, double *dy);
+void solve_B(Ctx &c, pevsl_Data *p, const double *dx, double *dy);
+void solve_LT(Ctx &c, pevsl_Data *p, const double *dx, double *dy);
+double gen_dot(Ctx &c, int n, const double *x, const double *y);
+// ngs passes of v -= N(:,0:k-1) (T(:,0:k-1)^T v)
+void cgs2(Ctx &c, int n, int k, int ngs, const double *T, const double *N, size_t ld,
+          double *v);
+// generalized filter y = p(A B^{-1}) v (device vectors)
+void cheb_filter_gen(Ctx &c, pevsl_Data *p, const pevsl_Polparams *pol, const double *dv,
+                     double *dy);
+// generalized KPM dot products dots[m*Mdeg + k]
+void kpm_moments_gen(Ctx &c, pevsl_Data *p, int Mdeg, int nvec, double ctr, double wid,
+                     std::vector<double> &dots);
+// generalized ChebLanNr for one slice (outputs as pEVSL_ChebLanNr)
+int cheblan_gen(Ctx &c, pevsl_Data *p, double *intv, int maxit, double tol,
+                pevsl_Parvec *vinit, pevsl_Polparams *pol, int *nevOut, double **lamo,
+                pevsl_Parvecs **Wo, double **reso, FILE *fstats);
 
 }  // namespace pevsl

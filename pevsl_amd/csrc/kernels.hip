@@ -789,6 +789,22 @@ k_dot_part_f32(int n, const float *__restrict__ x, const float *__restrict__ y,
   block_sum_store(acc, part + blockIdx.x);
 }
 
+// v_{k+1} = t (v_{k+1} - cc v_k) [- v_{k-1}];  y += mu v_{k+1} (y may be
+// null): the vector part of a Chebyshev step whose product was formed
+// separately (the reference's AXPY/SCAL sequence, SRC/chebpol.c:550-555)
+__global__ void k_cheb_update(int n, const double *__restrict__ vk,
+                              const double *__restrict__ vkm1, double *__restrict__ vkp1,
+                              double *__restrict__ y, double cc, double t, double mu) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  double w = vkp1[i] + (-cc) * vk[i];
+  w = w * t;
+  if (vkm1) w = w + (-1.0) * vkm1[i];
+  vkp1[i] = w;
+  if (y) y[i] = y[i] + mu * w;
+}
+
 __global__ void k_scale_f32(int n, float *__restrict__ x, double hs) {
   const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
   if (i < (size_t)n) x[i] = (float)((double)x[i] * hs);
@@ -1688,6 +1704,13 @@ void scale_vec(int n, double *x, const double *inv_scale_dev, double hostscale,
   if (n == 0) return;
   hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(kThreads), 0, s, n, x,
                      inv_scale_dev, hostscale);
+}
+
+void cheb_update(int n, const double *vk, const double *vkm1, double *vkp1, double *y,
+                 double cc, double t, double mu, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cheb_update, dim3(grid_for((size_t)n)), dim3(kThreads), 0, s, n, vk,
+                     vkm1, vkp1, y, cc, t, mu);
 }
 
 void axpy_vec(int n, double a, const double *x, double *y, hipStream_t s) {

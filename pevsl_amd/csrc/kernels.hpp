@@ -120,6 +120,10 @@ int dot_part_blocks(int n);
 void scale_vec(int n, double *x, const double *inv_scale_dev, double hostscale,
                hipStream_t s);
 void axpy_vec(int n, double a, const double *x, double *y, hipStream_t s);
+// vkp1 = t (vkp1 - cc vk) - vkm1 (vkm1 may be null);  y += mu vkp1 (y may be
+// null): the vector part of a Chebyshev step with a separately formed product
+void cheb_update(int n, const double *vk, const double *vkm1, double *vkp1, double *y,
+                 double cc, double t, double mu, hipStream_t s);
 void copy_f64_to_f32(size_t n, const double *x, float *y, hipStream_t s);
 void copy_f32_to_f64(size_t n, const float *x, double *y, hipStream_t s);
 void fill_zero_f64(size_t n, double *x, hipStream_t s);

@@ -1131,6 +1131,26 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs, int *max
     c.h_x.assign((size_t)pevsl->n, 0.0);
     c.h_y.assign((size_t)pevsl->n, 0.0);
   }
+  if (pevsl->ifGenEv) {
+    // generalized problem: one slice after the other on the device
+    c.info.clear();
+    for (int s = 0; s < nslices; s++) {
+      double *iv = intvs + 4 * (size_t)s;
+      pevsl_Parvecs *W = NULL;
+      if (iv[0] >= iv[1] || iv[0] >= iv[3] || iv[1] <= iv[2]) {
+        nevOut[s] = 0;
+        lamo[s] = reso[s] = NULL;
+        if (Wo) Wo[s] = NULL;
+        c.info.push_back(pevsl_LanInfo{});
+        continue;
+      }
+      const int rc = cheblan_gen(c, pevsl, iv, maxits[s], tol, vinit, &pols[s], &nevOut[s],
+                                 &lamo[s], Wo ? &W : NULL, &reso[s], fstats);
+      if (Wo) Wo[s] = W;
+      if (rc) return rc;
+    }
+    return 0;
+  }
   BatchSolver bs(c, pevsl, nslices);
   const int rc = bs.run(intvs, maxits, tol, vinit, pols, fstats);
   if (rc == -1) {
@@ -1158,7 +1178,10 @@ int pEVSL_ChebAv(pevsl_Data *pevsl, pevsl_Polparams *pol, pevsl_Parvec *v,
   double *d = c.cb_x.ensure((size_t)2 * n);
   PV_HIP(hipMemcpyAsync(d, v->data, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
   const double t0 = wtime();
-  cheb_filter_col(c, pol, d, d + n);
+  if (pevsl->ifGenEv)
+    cheb_filter_gen(c, pevsl, pol, d, d + n);
+  else
+    cheb_filter_col(c, pol, d, d + n);
   PV_HIP(hipMemcpyAsync(y->data, d + n, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
   PV_HIP(hipStreamSynchronize(c.stream));
   pevsl->stats->t_polAv += wtime() - t0;

@@ -1,5 +1,6 @@
 // lantrbnd.cpp -- spectrum bounds by thick-restart Lanczos
-// (pEVSL_LanTrbounds, SRC/lantrbnd.c:31-408), standard problem.
+// (pEVSL_LanTrbounds, SRC/lantrbnd.c:31-408), standard and generalized
+// problems.
 //
 // The restart basis V (lanm + 1 FP64 columns) lives in HBM and uses the
 // batched Lanczos kernels with one slice (kernels.hip k_lan_*): products with
@@ -78,6 +79,165 @@ void jacobi_eig(int n, std::vector<double> &A, std::vector<double> &w, std::vect
   Z.swap(Zs);
 }
 
+// Generalized problem (SRC/lantrbnd.c ifGenEv branches): bases V and
+// Z = B V in HBM, B-inner products, B-solves through the instance's solve,
+// CGS_DGKS2 by rocBLAS DGEMVs; T and the bounds as in the standard problem.
+int lan_trbounds_gen(pevsl_Data *pevsl, Ctx &c, int lanm, int maxit, double tol,
+                     pevsl_Parvec *vinit, int bndtype, double *lammin, double *lammax,
+                     FILE *fstats) {
+  const int n = pevsl->n, lanm1 = lanm + 1;
+  const int min_inner_step = 5;
+  const hipStream_t st = c.stream;
+  const size_t ld = ((size_t)n + 63) / 64 * 64;
+  DBuf<double> Vb, Zb, rb, sb;
+  double *V = Vb.ensure(ld * lanm1), *Z = Zb.ensure(ld * lanm1);
+  double *rv = rb.ensure(4 * ld), *sv = sb.ensure(2 * (size_t)lanm1 + 8);
+  auto vcol = [&](int j) { return V + (size_t)j * ld; };
+  auto zcol = [&](int j) { return Z + (size_t)j * ld; };
+  auto dot = [&](const double *x, const double *y) { return gen_dot(c, n, x, y); };
+  const double one = 1.0, zero = 0.0, mone = -1.0;
+  // V(:,0) = vinit / |vinit|_B, Z(:,0) = B V(:,0)
+  PV_HIP(hipMemcpyAsync(vcol(0), vinit->data, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  apply_B(c, pevsl, vcol(0), zcol(0));
+  double t = 1.0 / std::sqrt(dot(vcol(0), zcol(0)));
+  scale_vec(n, zcol(0), nullptr, t, st);
+  scale_vec(n, vcol(0), nullptr, t, st);
+  // a random vector B-orthogonalized against V(:,0:k-1) into (V, Z)(:,k),
+  // unit B-norm (lucky breakdown)
+  auto restart_random = [&](int k) {
+    rand_vectors(c, n, 1, vcol(k), ld);
+    cgs2(c, n, k, NGS_MAX, Z, V, ld, vcol(k));
+    apply_B(c, pevsl, vcol(k), zcol(k));
+    const double ib = 1.0 / std::sqrt(dot(vcol(k), zcol(k)));
+    scale_vec(n, vcol(k), nullptr, ib, st);
+    scale_vec(n, zcol(k), nullptr, ib, st);
+  };
+  std::vector<double> T((size_t)lanm1 * lanm1, 0.0), Rval((size_t)lanm1, 0.0), Tk, Ev, w;
+  auto Tat = [&](int r, int cc) -> double & { return T[(size_t)cc * lanm1 + r]; };
+  double s[3] = {0.0, 0.0, 0.0}, lmin = 0.0, lmax = 0.0, t1 = 0.0, t2 = 0.0;
+  int it = 0, trlen = 0, rc = 0;
+  while (it < maxit) {
+    double wn = 0.0, beta = 0.0;
+    int nwn = 0;
+    int k = trlen > 0 ? trlen + 1 : 0;
+    if (k + min_inner_step > lanm1) {
+      fprintf(stderr, "Krylov dim too small for this problem. Try a larger dim\n");
+      rc = -1;
+      break;
+    }
+    if (trlen > 0) {
+      const int k1 = k - 1;
+      double *znew = zcol(k1 + 1), *vnew = vcol(k1 + 1);
+      apply_A(c, vcol(k1), znew);
+      for (int i = 0; i < trlen; i++) {
+        Tat(i, i) = Rval[(size_t)i];
+        wn += std::fabs(Rval[(size_t)i]);
+      }
+      s[k1] = dot(vcol(k1), znew);
+      // znew -= Z(:,0:k-1) s(0:k-1)
+      PV_HIP(hipMemcpyAsync(sv, s, sizeof(double) * k, hipMemcpyHostToDevice, st));
+      PV_BLAS(rocblas_dgemv(c.blas, rocblas_operation_none, n, k, &mone, Z, (int)ld, sv, 1,
+                            &one, znew, 1));
+      for (int i = 0; i < k1; i++) {
+        Tat(i, trlen) = s[i];
+        Tat(trlen, i) = s[i];
+        wn += 2.0 * std::fabs(s[i]);
+      }
+      Tat(trlen, trlen) = s[k1];
+      wn += std::fabs(s[k1]);
+      solve_B(c, pevsl, znew, vnew);
+      beta = std::sqrt(dot(vnew, znew));
+      wn += 2.0 * beta;
+      nwn += 3 * k;
+      if (beta * nwn < orthTol * wn) {
+        restart_random(k1 + 1);
+        beta = 0.0;
+      } else {
+        scale_vec(n, vnew, nullptr, 1.0 / beta, st);
+        scale_vec(n, znew, nullptr, 1.0 / beta, st);
+      }
+      Tat(k, k1) = beta;
+      Tat(k1, k) = beta;
+    }
+    const double *zold = k > 0 ? zcol(k - 1) : nullptr;
+    while (k < lanm && it < maxit) {
+      k++;
+      double *v = vcol(k - 1), *z = zcol(k - 1), *vnew = vcol(k), *znew = zcol(k);
+      apply_A(c, v, znew);
+      it++;
+      if (zold) axpy_vec(n, -beta, zold, znew, st);
+      const double alpha = dot(v, znew);
+      Tat(k - 1, k - 1) = alpha;
+      wn += std::fabs(alpha);
+      axpy_vec(n, -alpha, z, znew, st);
+      cgs2(c, n, k, NGS_MAX, V, Z, ld, znew);
+      solve_B(c, pevsl, znew, vnew);
+      beta = std::sqrt(dot(vnew, znew));
+      wn += 2.0 * beta;
+      nwn += 3;
+      zold = z;
+      if (beta * nwn < orthTol * wn) {
+        if (fstats) fprintf(fstats, "it %4d: Lucky breakdown, beta = %.15e\n", it, beta);
+        restart_random(k);
+        beta = 0.0;
+      } else {
+        scale_vec(n, vnew, nullptr, 1.0 / beta, st);
+        scale_vec(n, znew, nullptr, 1.0 / beta, st);
+      }
+      Tat(k, k - 1) = beta;
+      Tat(k - 1, k) = beta;
+    }
+    Tk.assign((size_t)k * k, 0.0);
+    for (int cc = 0; cc < k; cc++)
+      for (int r = 0; r < k; r++) Tk[(size_t)cc * k + r] = Tat(r, cc);
+    jacobi_eig(k, Tk, w, Ev);
+    for (int i = 0; i < k; i++) Rval[(size_t)i] = w[(size_t)i];
+    s[0] = beta * Ev[(size_t)k - 1];
+    s[1] = beta * Ev[(size_t)(k - 1) * k + (k - 1)];
+    if (bndtype <= 1) {
+      t1 = std::fabs(s[0]);
+      t2 = std::fabs(s[1]);
+    } else {
+      t1 = 2.0 * s[0] * s[0] / (Rval[1] - Rval[0]);
+      t2 = 2.0 * s[1] * s[1] / (Rval[(size_t)k - 1] - Rval[(size_t)k - 2]);
+    }
+    lmin = Rval[0] - t1;
+    lmax = Rval[(size_t)k - 1] + t2;
+    // extreme Ritz vectors and their B-images become (V, Z)(:, 0:1)
+    std::vector<double> e2((size_t)2 * lanm1, 0.0);
+    std::copy(Ev.begin(), Ev.begin() + k, e2.begin());
+    std::copy(Ev.begin() + (long)(k - 1) * k, Ev.begin() + (long)k * k, e2.begin() + lanm1);
+    PV_HIP(hipMemcpyAsync(sv, e2.data(), sizeof(double) * 2 * lanm1, hipMemcpyHostToDevice, st));
+    for (int j = 0; j < 2; j++) {
+      PV_BLAS(rocblas_dgemv(c.blas, rocblas_operation_none, n, k, &one, V, (int)ld,
+                            sv + (size_t)j * lanm1, 1, &zero, rv + (size_t)j * ld, 1));
+      PV_BLAS(rocblas_dgemv(c.blas, rocblas_operation_none, n, k, &one, Z, (int)ld,
+                            sv + (size_t)j * lanm1, 1, &zero, rv + (size_t)(2 + j) * ld, 1));
+    }
+    trlen = 2;
+    for (int j = 0; j < 2; j++) {
+      PV_HIP(hipMemcpyAsync(vcol(j), rv + (size_t)j * ld, sizeof(double) * ld,
+                            hipMemcpyDeviceToDevice, st));
+      PV_HIP(hipMemcpyAsync(zcol(j), rv + (size_t)(2 + j) * ld, sizeof(double) * ld,
+                            hipMemcpyDeviceToDevice, st));
+    }
+    Rval[1] = Rval[(size_t)k - 1];
+    if (fstats)
+      fprintf(fstats, "it %4d, k %3d: ritz %.15e %.15e, t1,t2 %e %e, res %.15e %.15e\n", it, k,
+              Rval[0], Rval[1], t1, t2, std::fabs(s[0]), std::fabs(s[1]));
+    if (t1 + t2 < tol * (std::fabs(lmin) + std::fabs(lmax))) break;
+    std::fill(T.begin(), T.end(), 0.0);
+    PV_HIP(hipMemcpyAsync(vcol(trlen), vcol(k), sizeof(double) * ld, hipMemcpyDeviceToDevice,
+                          st));
+    PV_HIP(hipMemcpyAsync(zcol(trlen), zcol(k), sizeof(double) * ld, hipMemcpyDeviceToDevice,
+                          st));
+  }
+  PV_HIP(hipStreamSynchronize(st));
+  *lammin = lmin;
+  *lammax = lmax;
+  return rc;
+}
+
 }  // namespace
 }  // namespace pevsl
 
@@ -89,13 +249,18 @@ int pEVSL_LanTrbounds(pevsl_Data *pevsl, int lanm, int maxit, double tol, pevsl_
                       int bndtype, double *lammin, double *lammax, FILE *fstats) {
   const double tms = wtime();
   if (!pevsl || !vinit || !lammin || !lammax || lanm < 1) return -1;
-  if (pevsl->ifGenEv) return PEVSL_NOT_IMPLEMENT;  // generalized problem: out of scope
   Ctx &c = ctx_of(pevsl);
   const int n = pevsl->n, N = pevsl->N;
   lanm = std::min(lanm, N);
   const int lanm1 = lanm + 1;
   if (lanm == N) maxit = std::min(maxit, N);
   if (fstats) fprintf(fstats, " LanTR for bounds: dim %d, maxits %d\n", lanm, maxit);
+  if (pevsl->ifGenEv) {
+    const int rc = lan_trbounds_gen(pevsl, c, lanm, maxit, tol, vinit, bndtype, lammin,
+                                    lammax, fstats);
+    pevsl->stats->t_eigbounds += wtime() - tms;
+    return rc;
+  }
   const int min_inner_step = 5;
   const hipStream_t st = c.stream;
   // basis: one chunk of >= lanm + 1 columns, rows [n, ld) zero

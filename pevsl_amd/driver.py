@@ -626,3 +626,115 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
         M.free()
     out["t_total"] = time.time() - t0
     return out
+
+
+SVFUNC = C.CFUNCTYPE(None, capi.c_double_p, capi.c_double_p, C.c_void_p)
+
+
+def chain_laplacian(n):
+    """tridiag(-1, 4, -1) of order n: the B of TESTS/Gen_Lap/LapPLanN.c
+    (ParcsrLaplace(n, 1, 1): the stencil of TESTS/COMMON/lapl.c:4-70 on an
+    n x 1 x 1 grid, whose diagonal is 4 since nz = 1); spectrum in (2, 6)."""
+    return laplacian3d(n, 1, 1)
+
+
+class HostSolve:
+    """A B-solve (SVFunc) on the host, kept alive with its callback: the
+    pattern of a caller-supplied direct solver (the reference's tests use
+    MUMPS).  solve(b) -> x with B x = b."""
+
+    def __init__(self, solve):
+        self._solve = solve
+
+        def cb(pb, px, data):
+            n = self.n
+            b = np.ctypeslib.as_array(pb, shape=(n,))
+            x = np.ctypeslib.as_array(px, shape=(n,))
+            x[:] = self._solve(b.copy())
+        self.n = 0
+        self.func = SVFUNC(cb)
+
+    def ptr(self):
+        return C.cast(self.func, C.c_void_p)
+
+
+def gen_lap_run(nx, ny, nz, a, b, nslices, bsol="lspol", nvec=60, Mdeg=50, tol=1e-8,
+                lspol_deg=50, lspol_tol=1e-12, verbose=False, log=None):
+    """End-to-end run of TESTS/Gen_Lap/LapPLanN.c (KPM branch): A = 3-D
+    Laplacian, B = chain_laplacian(n) (spectrum in (2, 6)), L^T solve by
+    pEVSL_SetupLSPolSqrt(50, 1e-6, 2, 6) as the reference, B-solve by the
+    library's LSPol approximation of B^{-1} on [2, 6] ("lspol", on the
+    device) or an exact host Cholesky solve ("host", the place of the
+    reference's MUMPS solve); bounds by LanTrbounds(50, 200, 1e-8), KPM
+    (Mdeg 50, nvec 60), SpslicerKpm(npts = 10 ecount), FindPol (damping 2,
+    thresholds 0.6/0.15, max_deg 3000), ChebLanNr per slice with
+    mlan = min(max(5 nev, 300), n)."""
+    import scipy.linalg as sla
+    lib = capi.load()
+    n = nx * ny * nz
+    MA = Matrix(lib, *laplacian3d(nx, ny, nz))
+    MB = Matrix(lib, *chain_laplacian(n))
+    inst = Instance(lib, MA)
+    keep = []
+    lt = C.c_void_p()
+    bdata = C.c_void_p()
+    out = {"n": n}
+    try:
+        lib.pEVSL_SetBParcsr(inst.p, C.byref(MB.A))
+        if bsol == "lspol":
+            lib.pEVSL_SetupLSPolRec(lspol_deg, lspol_tol, 2.0, 6.0, C.byref(MB.A),
+                                    C.byref(bdata))
+            lib.pEVSL_SetBSol(inst.p, C.cast(lib.pEVSL_LSPolSol, C.c_void_p), bdata)
+        else:
+            import scipy.sparse as sp
+            ia, ja, av = chain_laplacian(n)
+            Bd = sp.csr_matrix((av, ja, ia), shape=(n, n)).toarray()
+            cho = sla.cho_factor(Bd)
+            hs = HostSolve(lambda r: sla.cho_solve(cho, r))
+            hs.n = n
+            keep.append(hs)
+            lib.pEVSL_SetBSol(inst.p, hs.ptr(), None)
+        lib.pEVSL_SetupLSPolSqrt(50, 1e-6, 2.0, 6.0, C.byref(MB.A), C.byref(lt))
+        lib.pEVSL_SetLTSol(inst.p, C.cast(lib.pEVSL_LSPolSol, C.c_void_p), lt)
+        lib.pEVSL_SetGenEig(inst.p)
+        vinit = rand_vinit(lib, n)
+        t0 = time.time()
+        rc, lmin, lmax = inst.lan_trbounds(50, 200, 1e-8, vinit, 1)
+        if rc:
+            raise RuntimeError("LanTrbounds error %d" % rc)
+        xintv = np.array([a, b, lmin, lmax])
+        rc, mu, ecount = inst.kpmdos(Mdeg, 1, nvec, xintv)
+        if rc:
+            raise RuntimeError("kpmdos error %d" % rc)
+        rc, sli = spslicer_kpm(lib, mu, xintv, nslices, int(10 * ecount))
+        if rc:
+            raise RuntimeError("spslicer error %d" % rc)
+        out.update(bounds=[lmin, lmax], ecount=ecount, sli=sli, mu=mu, t_dos=time.time() - t0)
+        nev = int(1 + ecount / float(nslices)) + 2
+        mlan = min(max(5 * nev, 300), n)
+        res = []
+        for s in range(nslices):
+            iv = [sli[s], sli[s + 1], lmin, lmax]
+            pol, rcp = find_pol(lib, iv, damping=2, thresh_int=0.6, thresh_ext=0.15,
+                                max_deg=3000)
+            rc, o = cheblan_batch(inst, [iv], [mlan], tol, vinit, [pol], False, verbose)
+            if rc:
+                raise RuntimeError("ChebLanNr error %d" % rc)
+            o = o[0]
+            res.append({"slice": s, "intv": iv[:2], "deg": pol.deg, "nev": len(o["lam"]),
+                        "lam": np.sort(o["lam"]), "res": o["res"], "info": o["info"]})
+            if log:
+                log("slice %d: deg %d its %d nev %d" % (s, pol.deg, o["info"]["its"],
+                                                       len(o["lam"])))
+            lib.pEVSL_FreePol(C.byref(pol))
+        out["slices"] = res
+        out["t_total"] = time.time() - t0
+    finally:
+        inst.close()
+        if bdata:
+            lib.pEVSL_LSPolFree(bdata)
+        if lt:
+            lib.pEVSL_LSPolFree(lt)
+        MA.free()
+        MB.free()
+    return out

@@ -189,12 +189,14 @@ def test_coo_csr_roundtrip_and_sortrow(lib):
     lib.pEVSL_FreeCsr(C.byref(csr))
 
 
-def test_generalized_setters_report_not_implemented(lib):
-    assert lib.pEVSL_SetGenEig(None) == capi.PEVSL_NOT_IMPLEMENT
-    assert lib.pEVSL_SetBSol(None, None, None) == capi.PEVSL_NOT_IMPLEMENT
-    # LanTrbounds is implemented (standard problem); missing arguments are an
-    # error, not a crash
+def test_missing_instance_is_an_error(lib):
+    # the generalized setters and LanTrbounds reject a missing instance
+    assert lib.pEVSL_SetGenEig(None) == -1
+    assert lib.pEVSL_SetBSol(None, None, None) == -1
+    assert lib.pEVSL_SetLTSol(None, None, None) == -1
+    assert lib.pEVSL_SetBMatvec(None, None, None) == -1
     assert lib.pEVSL_LanTrbounds(None, 10, 10, 1e-8, None, 0, None, None, None) == -1
+    assert lib.pEVSL_LSPolInfo(None, None, None, None, None) == -1
 
 
 @pytest.mark.parametrize("seed", [0, 1, 7, 123456789])

@@ -1,0 +1,196 @@
+"""GPU tests of the generalized problem A x = lambda B x through the C-ABI:
+the device LSPol solves, the generalized ChebAv and KPM against the dense
+oracle (oracle/gen_oracle.py) on the same inputs, LanTrbounds against the
+dense pencil spectrum, and the TESTS/Gen_Lap/LapPLanN.c flow with exact
+per-slice counts.  Tolerances are written per test."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+
+from oracle import gen_oracle as go
+from oracle import oracle as orc
+from pevsl_amd import capi, driver
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print("\n[gpu case] " + msg, flush=True)
+
+
+def dense(ia, ja, a, n):
+    return sp.csr_matrix((a, ja, ia), shape=(n, n)).toarray()
+
+
+class Gen:
+    """A = 3-D Laplacian, B = chain Laplacian (the Gen_Lap pair) on the
+    device, an instance in generalized mode with B-solve `bsol` ("host":
+    exact Cholesky callback; "lspol": the library's LSPol B^{-1} on [2, 6])
+    and L^T solve by LSPolSqrt(50, 1e-6, 2, 6)."""
+
+    def __init__(self, lib, nx, ny, nz, bsol):
+        self.lib = lib
+        n = nx * ny * nz
+        self.n = n
+        self.Acsr = driver.laplacian3d(nx, ny, nz)
+        self.Bcsr = driver.chain_laplacian(n)
+        self.A = dense(*self.Acsr, n)
+        self.B = dense(*self.Bcsr, n)
+        self.MA = driver.Matrix(lib, *self.Acsr)
+        self.MB = driver.Matrix(lib, *self.Bcsr)
+        self.inst = driver.Instance(lib, self.MA)
+        lib.pEVSL_SetBParcsr(self.inst.p, C.byref(self.MB.A))
+        cho = sla.cho_factor(self.B)
+        self.exact = lambda r: sla.cho_solve(cho, r)
+        self.bdata = C.c_void_p()
+        self.hs = None
+        if bsol == "lspol":
+            lib.pEVSL_SetupLSPolRec(50, 1e-12, 2.0, 6.0, C.byref(self.MB.A),
+                                    C.byref(self.bdata))
+            lib.pEVSL_SetBSol(self.inst.p, C.cast(lib.pEVSL_LSPolSol, C.c_void_p), self.bdata)
+            mu, cc, dd = go.lspol_coeffs(lambda x: 1.0 / x, 2.0, 6.0, 50, 1e-12)
+            self.solveB = lambda r: go.lspol_apply(self.B, mu, cc, dd, r)
+        else:
+            self.hs = driver.HostSolve(self.exact)
+            self.hs.n = n
+            lib.pEVSL_SetBSol(self.inst.p, self.hs.ptr(), None)
+            self.solveB = self.exact
+        self.lt = C.c_void_p()
+        lib.pEVSL_SetupLSPolSqrt(50, 1e-6, 2.0, 6.0, C.byref(self.MB.A), C.byref(self.lt))
+        lib.pEVSL_SetLTSol(self.inst.p, C.cast(lib.pEVSL_LSPolSol, C.c_void_p), self.lt)
+        mus, ccs, dds = go.lspol_coeffs(lambda x: 1.0 / math.sqrt(x), 2.0, 6.0, 50, 1e-6)
+        self.solveLT = lambda r: go.lspol_apply(self.B, mus, ccs, dds, r)
+        assert lib.pEVSL_SetGenEig(self.inst.p) == 0
+        self.lam = sla.eigh(self.A, self.B, eigvals_only=True)
+
+    def close(self):
+        self.inst.close()
+        if self.bdata:
+            self.lib.pEVSL_LSPolFree(self.bdata)
+        self.lib.pEVSL_LSPolFree(self.lt)
+        self.MA.free()
+        self.MB.free()
+
+
+def test_lspol_solves_on_device(lib):
+    banner("LSPol B^{-1} and B^{-1/2} solves, chain Laplacian n=3000")
+    n = 3000
+    ia, ja, a = driver.chain_laplacian(n)
+    M = driver.Matrix(lib, ia, ja, a)
+    Bs = sp.csr_matrix((a, ja, ia), shape=(n, n))
+    b = np.random.default_rng(1).standard_normal(n)
+    try:
+        for which, f, deg, tol in (("rec", lambda x: 1.0 / x, 50, 1e-12),
+                                   ("sqrt", lambda x: 1.0 / math.sqrt(x), 50, 1e-6)):
+            vd = C.c_void_p()
+            setup = lib.pEVSL_SetupLSPolRec if which == "rec" else lib.pEVSL_SetupLSPolSqrt
+            setup(deg, tol, 2.0, 6.0, C.byref(M.A), C.byref(vd))
+            x = np.zeros(n)
+            lib.pEVSL_LSPolSol(b.ctypes.data_as(capi.c_double_p),
+                               x.ctypes.data_as(capi.c_double_p), vd)
+            lib.pEVSL_LSPolFree(vd)
+            mu, cc, dd = go.lspol_coeffs(f, 2.0, 6.0, deg, tol)
+            ref = go.lspol_apply(Bs, mu, cc, dd, b)
+            # same series, FP64 on both sides: relative 1e-12
+            assert np.linalg.norm(x - ref) < 1e-12 * np.linalg.norm(ref)
+            if which == "rec":
+                ex = sp.linalg.spsolve(Bs.tocsc(), b)
+                assert np.linalg.norm(x - ex) < 1e-10 * np.linalg.norm(ex)
+    finally:
+        M.free()
+
+
+@pytest.mark.parametrize("bsol", ["host", "lspol"])
+def test_chebav_generalized_matches_oracle(lib, bsol):
+    banner("generalized ChebAv 8^3, B-solve %s" % bsol)
+    g = Gen(lib, 8, 8, 8, bsol)
+    try:
+        lmin, lmax = g.lam[0] * 0.99, g.lam[-1] * 1.01
+        pol, rc = driver.find_pol(lib, [1.5, 2.0, lmin, lmax])
+        assert rc == 0
+        mu = driver.pol_mu(pol)
+        v = np.random.default_rng(2).uniform(-1, 1, g.n)
+        y = np.zeros(g.n)
+        X, Y = capi.Parvec(), capi.Parvec()
+        lib.pEVSL_ParvecCreateShell(g.n, g.n, 0, 0, C.byref(X), v.ctypes.data_as(capi.c_double_p))
+        lib.pEVSL_ParvecCreateShell(g.n, g.n, 0, 0, C.byref(Y), y.ctypes.data_as(capi.c_double_p))
+        assert lib.pEVSL_ChebAv(g.inst.p, C.byref(pol), C.byref(X), C.byref(Y), None) == 0
+        ref = go.cheb_av_gen(g.A, g.solveB, mu, pol.cc, pol.dd, v)
+        # FP64 on both sides, same recurrence: relative 1e-11
+        assert np.linalg.norm(y - ref) < 1e-11 * np.linalg.norm(ref), pol.deg
+        lib.pEVSL_FreePol(C.byref(pol))
+    finally:
+        g.close()
+
+
+def test_kpm_generalized_matches_oracle(lib):
+    banner("generalized KPM 8^3, exact B-solve, LSPol L^T solve")
+    g = Gen(lib, 8, 8, 8, "host")
+    try:
+        Mdeg, nvec = 30, 6
+        lmin, lmax = g.lam[0] * 0.99, g.lam[-1] * 1.01
+        xintv = np.array([1.5, 2.5, lmin, lmax])
+        rc, mu, ecount = g.inst.kpmdos(Mdeg, 1, nvec, xintv)
+        assert rc == 0
+        # the same random vectors: the library's stream equals libc
+        # srand(0)/rand() (pEVSL_Start seeds 0; KPM draws nvec x n values)
+        orc.srand(0)
+        vecs = np.array([orc.rand_vec(g.n) for _ in range(nvec)])
+        ctr, wid = (lmax + lmin) / 2, (lmax - lmin) / 2
+        dots = go.kpm_gen_dots(g.A, g.B, g.solveB, g.solveLT, vecs, Mdeg, ctr, wid)
+        jac = orc.dampcf(Mdeg, 1)
+        jac[0] = 1.0
+        ref = np.zeros(Mdeg + 1)
+        ref[0] = nvec * jac[0]
+        ref[1:] = 2.0 * jac[1:] * dots.sum(0)
+        ref /= nvec * np.pi
+        # FP64 both sides; the L^T solve is the same series: relative 1e-10
+        assert np.max(np.abs(mu - ref)) < 1e-10 * np.max(np.abs(ref))
+        exact = np.sum((g.lam >= 1.5) & (g.lam <= 2.5))
+        assert abs(ecount - exact) < 0.35 * exact  # 6 vectors: stochastic estimate
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("bsol", ["host", "lspol"])
+def test_lantrbounds_generalized(lib, bsol):
+    banner("generalized LanTrbounds 8^3, B-solve %s" % bsol)
+    g = Gen(lib, 8, 8, 8, bsol)
+    try:
+        vinit = driver.rand_vinit(lib, g.n)
+        rc, lo, hi = g.inst.lan_trbounds(50, 500, 1e-8, vinit, 1)
+        assert rc == 0
+        scale = max(abs(g.lam[0]), abs(g.lam[-1]))
+        # bounds enclose the pencil's spectrum and are tight to the tolerance
+        assert lo <= g.lam[0] + 1e-10 * scale and hi >= g.lam[-1] - 1e-10 * scale
+        assert abs(lo - g.lam[0]) < 1e-6 * scale and abs(hi - g.lam[-1]) < 1e-6 * scale
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("bsol", ["host", "lspol"])
+def test_gen_lap_flow_exact_counts(lib, bsol):
+    """TESTS/Gen_Lap/LapPLanN.c defaults (8^3, [1.5, 2.5], 4 slices): every
+    slice finds the dense pencil count, eigenvalues to 1e-10 relative,
+    residuals ||A u - lambda B u|| < 1e-8."""
+    banner("Gen_Lap flow 8^3 [1.5, 2.5] 4 slices, B-solve %s" % bsol)
+    r = driver.gen_lap_run(8, 8, 8, 1.5, 2.5, 4, bsol=bsol)
+    A = dense(*driver.laplacian3d(8, 8, 8), 512)
+    B = dense(*driver.chain_laplacian(512), 512)
+    lam = sla.eigh(A, B, eigvals_only=True)
+    lmin, lmax = r["bounds"]
+    assert lmin <= lam[0] + 1e-8 and lmax >= lam[-1] - 1e-8
+    total = 0
+    for s in r["slices"]:
+        lo, hi = s["intv"]
+        ex = lam[(lam >= lo) & (lam <= hi)]
+        assert s["nev"] == len(ex), (s["slice"], s["nev"], len(ex))
+        if len(ex):
+            assert np.max(np.abs(s["lam"] - ex) / np.abs(ex)) < 1e-10
+            assert np.all(np.asarray(s["res"]) < 1e-8)
+        total += s["nev"]
+    assert total == np.sum((lam >= r["sli"][0]) & (lam <= r["sli"][-1])) > 20
