@@ -23,6 +23,7 @@
 #include <time.h>
 
 typedef int MPI_Comm;
+typedef int MPI_Fint;  /* Fortran handle of a communicator */
 typedef int MPI_Datatype;
 typedef int MPI_Op;
 typedef int MPI_Request;
@@ -44,6 +45,7 @@ static inline size_t pevsl_mpi_size_of(MPI_Datatype t) {
   return t == MPI_INT ? sizeof(int) : (t == MPI_UNSIGNED_LONG ? sizeof(unsigned long) : sizeof(double));
 }
 
+static inline MPI_Comm MPI_Comm_f2c(MPI_Fint f) { return (MPI_Comm)f; }
 static inline int MPI_Init(int *argc, char ***argv) { (void)argc; (void)argv; return 0; }
 static inline int MPI_Finalize(void) { return 0; }
 static inline int MPI_Abort(MPI_Comm c, int code) { (void)c; exit(code); return code; }

@@ -100,3 +100,44 @@ def kpm_gen_dots(A, B, solveB, solveLT, vecs, Mdeg, ctr, wid):
             vkm1, vk = vk, vkp1
             out[m, k] = vk @ v
     return out
+
+
+def chebiter_solv1(B, b, lb, ub, deg):
+    """Chebyshev iteration, Templates form (EXTERNAL/ITERSOL/chebiter.c:
+    121-200): x_1 = (2/d) b, then p = r + bet p, x += alp p with
+    bet = (c alp / 2)^2, alp = 1/(d - bet), d and c the centre and half
+    width of [lb, ub].  Returns None for deg < 1 (x left untouched)."""
+    if deg < 1:
+        return None
+    d, c = (ub + lb) * 0.5, (ub - lb) * 0.5
+    alp = 2.0 / d
+    p = b.copy()
+    x = alp * p
+    r = b - B @ x
+    for _ in range(1, deg):
+        t = c * alp * 0.5
+        bet = t * t
+        alp = 1.0 / (d - bet)
+        p = bet * p + r
+        x = x + alp * p
+        r = b - B @ x
+    return x
+
+
+def chebiter_solv2(B, b, lb, ub, deg):
+    """Chebyshev iteration of Saad, Iterative Methods for Sparse Linear
+    Systems (2nd ed.), Algorithm 12.1 (EXTERNAL/ITERSOL/chebiter.c:204-270),
+    zero initial guess."""
+    theta, delta = (ub + lb) * 0.5, (ub - lb) * 0.5
+    sigma = theta / delta
+    rho = 1.0 / sigma
+    x = np.zeros_like(b)
+    r = b.copy()
+    d = r / theta
+    for _ in range(deg):
+        x = x + d
+        r = r - B @ d
+        rho1 = 1.0 / (2.0 * sigma - rho)
+        d = rho1 * rho * d + (2.0 * rho1 / delta) * r
+        rho = rho1
+    return x

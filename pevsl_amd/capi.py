@@ -226,6 +226,46 @@ SIGNATURES = {
     "pEVSL_SetKeepVectors": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetBlockKernel": (C.c_int, [DataP, C.c_int]),
     "pEVSL_RandSeed": (None, [C.c_uint]),
+    "pEVSL_ChebIterSetup": (C.c_int, [C.c_double, C.c_double, C.c_int, C.POINTER(Parcsr),
+                                      C.POINTER(C.c_void_p)]),
+    "pEVSL_ChebIterSolv1": (None, [c_double_p, c_double_p, C.c_void_p]),
+    "pEVSL_ChebIterSolv2": (None, [c_double_p, c_double_p, C.c_void_p]),
+    "pEVSL_ChebIterFree": (None, [C.c_void_p]),
+    "pEVSL_ChebIterGetRes": (c_double_p, [C.c_void_p]),
+    "pEVSL_ChebIterStatsPrint": (None, [C.c_void_p, C.c_void_p]),
+    # Fortran 90 interface (include/pevsl_f90.h): all arguments by reference
+    "pevsl_start_f90_": (None, [C.c_void_p] * 2),
+    "pevsl_finish_f90_": (None, [C.c_void_p] * 1),
+    "pevsl_seta_parcsr_f90_": (None, [C.c_void_p] * 2),
+    "pevsl_setb_parcsr_f90_": (None, [C.c_void_p] * 2),
+    "pevsl_setprobsizes_f90_": (None, [C.c_void_p] * 4),
+    "pevsl_setamv_f90_": (None, [C.c_void_p] * 3),
+    "pevsl_setbmv_f90_": (None, [C.c_void_p] * 3),
+    "pevsl_setbsol_f90_": (None, [C.c_void_p] * 3),
+    "pevsl_set_stdeig_f90_": (None, [C.c_void_p] * 1),
+    "pevsl_set_geneig_f90_": (None, [C.c_void_p] * 1),
+    "pevsl_parcsrcreate_f90_": (None, [C.c_void_p] * 9),
+    "pevsl_parcsr_free_f90_": (None, [C.c_void_p] * 1),
+    "pevsl_parcsrmatvec_f90_": (None, [C.c_void_p] * 3),
+    "pevsl_bsv_f90_": (None, [C.c_void_p] * 3),
+    "pevsl_lanbounds_f90_": (None, [C.c_void_p] * 6),
+    "pevsl_findpol_f90_": (None, [C.c_void_p] * 4),
+    "pevsl_freepol_f90_": (None, [C.c_void_p] * 1),
+    "pevsl_cheblannr_f90_": (None, [C.c_void_p] * 5),
+    "pevsl_get_nev_f90_": (None, [C.c_void_p] * 2),
+    "pevsl_copy_result_f90_": (None, [C.c_void_p] * 4),
+    "pevsl_setup_chebiter_f90_": (None, [C.c_void_p] * 5),
+    "pevsl_chebiter_f90_": (None, [C.c_void_p] * 4),
+    "pevsl_chebiterstatsprint_f90_": (None, [C.c_void_p] * 1),
+    "pevsl_free_chebiterb_f90_": (None, [C.c_void_p] * 1),
+    "pevsl_setbsol_chebiter_f90_": (None, [C.c_void_p] * 3),
+    "pevsl_setup_lspolsqrt_f90_": (None, [C.c_void_p] * 6),
+    "pevsl_lspolsol_f90_": (None, [C.c_void_p] * 3),
+    "pevsl_free_lspol_f90_": (None, [C.c_void_p] * 1),
+    "pevsl_setltsol_lspol_f90_": (None, [C.c_void_p] * 2),
+    "pevsl_kpmdos_ecnt_f90_": (None, [C.c_void_p] * 7),
+    "pevsl_landos_ecnt_f90_": (None, [C.c_void_p] * 6),
+    "pevsl_set_sigmamult_f90_": (None, [C.c_void_p] * 2),
     "pEVSL_SetupLSPolSqrt": (None, [C.c_int, C.c_double, C.c_double, C.c_double,
                                     C.POINTER(Parcsr), C.POINTER(C.c_void_p)]),
     "pEVSL_SetupLSPolRec": (None, [C.c_int, C.c_double, C.c_double, C.c_double,

@@ -8,9 +8,10 @@
 //
 // B enters as a device CSR (SetBParcsr) or a host callback (SetBMatvec).  The
 // B- and L^T-solves are the reference's SVFunc host callbacks, called with
-// host copies of the vectors -- except that the library's own
-// pEVSL_LSPolSol is recognised and applied on the device, as the fused
-// Chebyshev step kernel on B's device CSR, with no host round trip.  The
+// host copies of the vectors -- except that the library's own solves
+// (pEVSL_LSPolSol: the fused Chebyshev step kernel on B's device CSR;
+// pEVSL_ChebIterSolv1/2: Chebyshev iteration) are recognised and applied on
+// the device, with no host round trip.  The
 // vectors, the two Lanczos bases V and Z = B V and the reorthogonalization
 // (rocBLAS DGEMVs) stay in HBM; host work is the O(k) Lanczos scalars and
 // the tridiagonal eigenvectors.
@@ -144,6 +145,11 @@ void solve_with(Ctx &c, pevsl_Data *p, SVFunc f, void *data, const double *dx, d
     const LSPolData &L = *static_cast<const LSPolData *>(data);
     if (L.n != p->n) fatal("LSPol data of another size", __FILE__, __LINE__);
     lspol_apply(L, dx, dy, c.gen_lsw.ensure((size_t)3 * p->n), c.stream);
+    return;
+  }
+  if (f == pEVSL_ChebIterSolv1 || f == pEVSL_ChebIterSolv2) {
+    if (chebiter_size(data) != p->n) fatal("ChebIter data of another size", __FILE__, __LINE__);
+    chebiter_device_solve(f, data, dx, dy, c.gen_lsw.ensure((size_t)3 * p->n), c.stream);
     return;
   }
   host_round_trip(c, p->n, f, data, dx, dy);

@@ -14,6 +14,7 @@
 extern "C" {
 #include "pevsl.h"
 #include "pevsl_amd.h"
+#include "pevsl_itsol.h"
 }
 
 namespace pevsl {
@@ -167,6 +168,11 @@ void cheb_filter_gen(Ctx &c, pevsl_Data *p, const pevsl_Polparams *pol, const do
 // generalized KPM dot products dots[m*Mdeg + k]
 void kpm_moments_gen(Ctx &c, pevsl_Data *p, int Mdeg, int nvec, double ctr, double wid,
                      std::vector<double> &dots);
+// y = f(x) on device vectors when f is one of the library's Chebyshev-
+// iteration solves (itsol.cpp); false otherwise.  wk: 3 n doubles.
+bool chebiter_device_solve(SVFunc f, void *data, const double *dx, double *dy, double *wk,
+                           hipStream_t s);
+int chebiter_size(void *data);
 // generalized ChebLanNr for one slice (outputs as pEVSL_ChebLanNr)
 int cheblan_gen(Ctx &c, pevsl_Data *p, double *intv, int maxit, double tol,
                 pevsl_Parvec *vinit, pevsl_Polparams *pol, int *nevOut, double **lamo,

@@ -659,12 +659,13 @@ class HostSolve:
 
 
 def gen_lap_run(nx, ny, nz, a, b, nslices, bsol="lspol", nvec=60, Mdeg=50, tol=1e-8,
-                lspol_deg=50, lspol_tol=1e-12, verbose=False, log=None):
+                lspol_deg=50, lspol_tol=1e-12, chebiter_deg=40, verbose=False, log=None):
     """End-to-end run of TESTS/Gen_Lap/LapPLanN.c (KPM branch): A = 3-D
     Laplacian, B = chain_laplacian(n) (spectrum in (2, 6)), L^T solve by
     pEVSL_SetupLSPolSqrt(50, 1e-6, 2, 6) as the reference, B-solve by the
     library's LSPol approximation of B^{-1} on [2, 6] ("lspol", on the
-    device) or an exact host Cholesky solve ("host", the place of the
+    device), Chebyshev iteration on [2, 6] ("chebiter", chebiter_deg steps,
+    on the device) or an exact host Cholesky solve ("host", the place of the
     reference's MUMPS solve); bounds by LanTrbounds(50, 200, 1e-8), KPM
     (Mdeg 50, nvec 60), SpslicerKpm(npts = 10 ecount), FindPol (damping 2,
     thresholds 0.6/0.15, max_deg 3000), ChebLanNr per slice with
@@ -678,6 +679,7 @@ def gen_lap_run(nx, ny, nz, a, b, nslices, bsol="lspol", nvec=60, Mdeg=50, tol=1
     keep = []
     lt = C.c_void_p()
     bdata = C.c_void_p()
+    cdata = C.c_void_p()
     out = {"n": n}
     try:
         lib.pEVSL_SetBParcsr(inst.p, C.byref(MB.A))
@@ -685,6 +687,9 @@ def gen_lap_run(nx, ny, nz, a, b, nslices, bsol="lspol", nvec=60, Mdeg=50, tol=1
             lib.pEVSL_SetupLSPolRec(lspol_deg, lspol_tol, 2.0, 6.0, C.byref(MB.A),
                                     C.byref(bdata))
             lib.pEVSL_SetBSol(inst.p, C.cast(lib.pEVSL_LSPolSol, C.c_void_p), bdata)
+        elif bsol == "chebiter":
+            lib.pEVSL_ChebIterSetup(2.0, 6.0, chebiter_deg, C.byref(MB.A), C.byref(cdata))
+            lib.pEVSL_SetBSol(inst.p, C.cast(lib.pEVSL_ChebIterSolv2, C.c_void_p), cdata)
         else:
             import scipy.sparse as sp
             ia, ja, av = chain_laplacian(n)
@@ -733,6 +738,8 @@ def gen_lap_run(nx, ny, nz, a, b, nslices, bsol="lspol", nvec=60, Mdeg=50, tol=1
         inst.close()
         if bdata:
             lib.pEVSL_LSPolFree(bdata)
+        if cdata:
+            lib.pEVSL_ChebIterFree(cdata)
         if lt:
             lib.pEVSL_LSPolFree(lt)
         MA.free()

@@ -18,7 +18,7 @@ GOLD = os.path.join(ROOT, "tests", "golden")
 
 def declared_functions():
     names = set()
-    for h in ("pevsl.h", "pevsl_amd.h"):
+    for h in ("pevsl.h", "pevsl_amd.h", "pevsl_itsol.h", "pevsl_f90.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for m in re.finditer(r"^\s*(?:int|void|double)\s+\*?\s*(\w+)\s*\(", txt, re.M):

@@ -119,3 +119,32 @@ def test_kpm_gen_dots_match_spectral_form():
         for k in range(12):
             tk = np.cos((k + 1) * np.arccos(np.clip(x, -1, 1)))
             assert abs(dots[m, k] - c @ tk) < 1e-11 * max(1.0, np.sum(c))
+
+
+@pytest.mark.parametrize("deg", [1, 3, 6])
+def test_chebiter_solv2_residual_polynomial(deg):
+    """After k steps from x_0 = 0, Chebyshev iteration leaves the residual
+    r_k = T_k((theta - B)/delta) / T_k(theta/delta) b (Saad 2003, sec. 12.3)."""
+    A, _ = _pencil(25, 11)
+    ev, U = np.linalg.eigh(A @ A.T + np.eye(25))
+    lb, ub = ev[0] * 0.9, ev[-1] * 1.1
+    B = U @ np.diag(ev) @ U.T
+    b = np.random.default_rng(12).standard_normal(25)
+    x = go.chebiter_solv2(B, b, lb, ub, deg)
+    theta, delta = 0.5 * (ub + lb), 0.5 * (ub - lb)
+    tk = np.polynomial.chebyshev.chebval((theta - ev) / delta, [0] * deg + [1])
+    tk0 = np.polynomial.chebyshev.chebval(theta / delta, [0] * deg + [1])
+    r = U @ ((tk / tk0) * (U.T @ b))
+    assert np.linalg.norm((b - B @ x) - r) < 1e-12 * np.linalg.norm(b)
+
+
+def test_chebiter_both_forms_converge_to_the_solve():
+    A, _ = _pencil(30, 13)
+    ev, U = np.linalg.eigh(A @ A.T / 30 + np.eye(30))
+    B = U @ np.diag(ev) @ U.T
+    b = np.random.default_rng(14).standard_normal(30)
+    ex = np.linalg.solve(B, b)
+    for f in (go.chebiter_solv1, go.chebiter_solv2):
+        x = f(B, b, ev[0], ev[-1], 200)
+        assert np.linalg.norm(x - ex) < 1e-12 * np.linalg.norm(ex)
+    assert go.chebiter_solv1(B, b, ev[0], ev[-1], 0) is None

@@ -172,7 +172,35 @@ def test_lantrbounds_generalized(lib, bsol):
         g.close()
 
 
-@pytest.mark.parametrize("bsol", ["host", "lspol"])
+@pytest.mark.parametrize("deg", [1, 5, 40])
+def test_chebiter_solves_match_oracle(lib, deg):
+    """pEVSL_ChebIterSolv1/2 (device, host arrays) against the oracle's two
+    forms on the same B and b: relative 1e-12; 40 steps on [2, 6] reach the
+    exact solve (contraction (sqrt(3)-1)/(sqrt(3)+1) per step)."""
+    banner("ChebIter solves, chain Laplacian n=3000, %d steps" % deg)
+    n = 3000
+    ia, ja, a = driver.chain_laplacian(n)
+    M = driver.Matrix(lib, ia, ja, a)
+    Bs = sp.csr_matrix((a, ja, ia), shape=(n, n))
+    b = np.random.default_rng(3).standard_normal(n)
+    cd = C.c_void_p()
+    try:
+        assert lib.pEVSL_ChebIterSetup(2.0, 6.0, deg, C.byref(M.A), C.byref(cd)) == 0
+        for solv, ref in ((lib.pEVSL_ChebIterSolv1, go.chebiter_solv1(Bs, b, 2.0, 6.0, deg)),
+                          (lib.pEVSL_ChebIterSolv2, go.chebiter_solv2(Bs, b, 2.0, 6.0, deg))):
+            x = np.zeros(n)
+            solv(b.ctypes.data_as(capi.c_double_p), x.ctypes.data_as(capi.c_double_p), cd)
+            assert np.linalg.norm(x - ref) < 1e-12 * np.linalg.norm(ref)
+            if deg == 40:
+                ex = sp.linalg.spsolve(Bs.tocsc(), b)
+                assert np.linalg.norm(x - ex) < 1e-10 * np.linalg.norm(ex)
+    finally:
+        if cd:
+            lib.pEVSL_ChebIterFree(cd)
+        M.free()
+
+
+@pytest.mark.parametrize("bsol", ["host", "lspol", "chebiter"])
 def test_gen_lap_flow_exact_counts(lib, bsol):
     """TESTS/Gen_Lap/LapPLanN.c defaults (8^3, [1.5, 2.5], 4 slices): every
     slice finds the dense pencil count, eigenvalues to 1e-10 relative,
