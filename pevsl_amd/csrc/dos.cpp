@@ -79,7 +79,11 @@ void kpm_moments(Ctx &c, int n, int Mdeg, int nvec, double ctr, double wid,
                  std::vector<double> &dots /* [nvec][Mdeg] */) {
   const DevCsr &A = *c.A;
   const hipStream_t s = c.stream;
-  const int B = nvec > 32 ? 64 : (nvec > 16 ? 32 : (nvec > 8 ? 16 : (nvec > 4 ? 8 : (nvec > 2 ? 4 : (nvec > 1 ? 2 : 1)))));
+  // uneven rows: nnz-balanced step, blocks of at most 32 vectors
+  const bool bal = A.max_row > kUnevenRow;
+  const int bmax = bal ? 32 : 64;
+  int B = 1;
+  while (B < nvec && B < bmax) B *= 2;
   const size_t blk = (size_t)n * B;
   DBuf<double> raw, part, dd;
   DBuf<T> v0, b0, b1, b2;
@@ -88,7 +92,8 @@ void kpm_moments(Ctx &c, int n, int Mdeg, int nvec, double ctr, double wid,
   b0.ensure(blk);
   b1.ensure(blk);
   b2.ensure(blk);
-  const int nrb = kpm_part_blocks(n, B);
+  int nrb = kpm_part_blocks(n, B);
+  if (bal) nrb = std::max(nrb, A.nrowblk_q[bal_level((int)sizeof(T), B)]);
   part.ensure((size_t)nrb * B);
   dd.ensure((size_t)Mdeg * B + (size_t)dot_part_blocks(n) + 8);
   DBuf<double *> cp;
@@ -119,7 +124,8 @@ void kpm_moments(Ctx &c, int n, int Mdeg, int nvec, double ctr, double wid,
     T *vk = b0.p, *vkm1 = b1.p, *vkp1 = b2.p;
     for (int k = 0; k < Mdeg; k++) {
       const double scal = (k == 0 ? 1.0 : 2.0) / wid;
-      const int nbk = kpm_step_blk<T>(A, B, vk, vkm1, vkp1, v0.p, ctr, scal, part.p, s);
+      const int nbk = bal ? kpm_step_bal<T>(A, B, vk, vkm1, vkp1, v0.p, ctr, scal, part.p, s)
+                          : kpm_step_blk<T>(A, B, vk, vkm1, vkp1, v0.p, ctr, scal, part.p, s);
       reduce_partials(part.p, nbk, B, dd.p + (size_t)k * B, s);
       T *t = vkm1;
       vkm1 = vk;

@@ -569,17 +569,20 @@ struct EpiClen {
 // gathers, so that 8 gathers per thread are in flight at once.
 constexpr int kBalUnroll = 8;
 
-template <typename T, int B, class Epi>
-__global__ void __launch_bounds__(kThreads)
-k_blk_bal(const int *__restrict__ rowblk, const int *__restrict__ ia,
-          const int *__restrict__ ja, const T *__restrict__ a, const T *__restrict__ x,
-          Epi epi) {
+// Both phases for row block rb: calls f(row, piece, acc) once per (row,
+// piece) of the block with the row's product sums for that piece, from the
+// thread threadIdx.x with threadIdx.x % Q == piece (phase 2 covers each
+// (row, piece) with one thread, the long-row path uses threads 0..Q-1).
+// prod: the block's LDS staging (NZ * B values).
+template <typename T, int B, class F>
+__device__ __forceinline__ void bal_rows(int rb, const int *__restrict__ rowblk,
+                                         const int *__restrict__ ia, const int *__restrict__ ja,
+                                         const T *__restrict__ a, const T *__restrict__ x,
+                                         T *prod, F &&f) {
 #pragma clang fp contract(off)
   using PC = Pieces<T, B>;
   constexpr int P = PC::P, Q = PC::Q, S = kThreads / Q, U = kBalUnroll;
   static_assert(PC::NZ == U * S, "one unrolled batch covers a staged block");
-  __shared__ T prod[PC::NZ * B];
-  const int rb = xcd_block(blockIdx.x, gridDim.x);
   const int r0 = rowblk[rb], r1 = rowblk[rb + 1];
   const int j0 = ia[r0], nz = ia[r1] - j0;
   const int piece = threadIdx.x % Q, slot = threadIdx.x / Q;
@@ -637,7 +640,7 @@ k_blk_bal(const int *__restrict__ rowblk, const int *__restrict__ ia,
         for (int w = 0; w < kThreads / 64; w++) sum += prod[w * B + piece * P + e];
         acc[e] = sum;
       }
-      epi(r0, piece, acc);
+      f(r0, piece, acc);
     }
     return;
   }
@@ -668,7 +671,65 @@ k_blk_bal(const int *__restrict__ rowblk, const int *__restrict__ ia,
 #pragma unroll
       for (int e = 0; e < P; e++) acc[e] += pv[e];
     }
-    epi(r, pc, acc);
+    f(r, pc, acc);
+  }
+}
+
+template <typename T, int B, class Epi>
+__global__ void __launch_bounds__(kThreads)
+k_blk_bal(const int *__restrict__ rowblk, const int *__restrict__ ia,
+          const int *__restrict__ ja, const T *__restrict__ a, const T *__restrict__ x,
+          Epi epi) {
+  __shared__ T prod[Pieces<T, B>::NZ * B];
+  bal_rows<T, B>(xcd_block(blockIdx.x, gridDim.x), rowblk, ia, ja, a, x, prod,
+                 [&](int r, int pc, const T (&acc)[Pieces<T, B>::P]) { epi(r, pc, acc); });
+}
+
+// KPM step on B interleaved vectors with the nnz-balanced products (uneven
+// rows): vkp1 = scal (A vk - ctr vk) - vkm1 and, per column, the partial dot
+// of vkp1 with v0 over the row block, part[rb * B + c] (rb = row block, so
+// the second-stage sum runs in row order whatever the dispatch order).
+template <typename T, int B>
+__global__ void __launch_bounds__(kThreads)
+k_kpm_bal(const int *__restrict__ rowblk, const int *__restrict__ ia,
+          const int *__restrict__ ja, const T *__restrict__ a, const T *__restrict__ vk,
+          const T *__restrict__ vkm1, T *__restrict__ vkp1, const T *__restrict__ v0, T ctr,
+          T scal, double *__restrict__ part) {
+#pragma clang fp contract(off)
+  using PC = Pieces<T, B>;
+  constexpr int P = PC::P, Q = PC::Q;
+  __shared__ T prod[PC::NZ * B];
+  const int rb = xcd_block(blockIdx.x, gridDim.x);
+  double dsum[P];
+#pragma unroll
+  for (int e = 0; e < P; e++) dsum[e] = 0.0;
+  bal_rows<T, B>(rb, rowblk, ia, ja, a, vk, prod, [&](int r, int pc, const T (&acc)[P]) {
+    const size_t o = (size_t)r * B + pc * P;
+    T v[P], vm[P], z[P], w[P];
+    ld_piece<T, P>(vk + o, v);
+    ld_piece<T, P>(vkm1 + o, vm);
+    ld_piece<T, P>(v0 + o, z);
+#pragma unroll
+    for (int e = 0; e < P; e++) {
+      w[e] = acc[e] + (-ctr) * v[e];
+      w[e] = w[e] * scal;
+      w[e] = w[e] + (T)(-1) * vm[e];
+      dsum[e] += (double)w[e] * (double)z[e];
+    }
+    st_piece<T, P>(vkp1 + o, w);
+  });
+  // per column: sum over the threads holding its piece, in thread order
+  __syncthreads();
+  double *red = reinterpret_cast<double *>(prod);
+  static_assert(kThreads * P * sizeof(double) <= PC::NZ * B * sizeof(T), "LDS reuse");
+#pragma unroll
+  for (int e = 0; e < P; e++) red[threadIdx.x * P + e] = dsum[e];
+  __syncthreads();
+  if (threadIdx.x < B) {
+    const int c = threadIdx.x, pc = c / P, e = c % P;
+    double sum = 0.0;
+    for (int t = pc; t < kThreads; t += Q) sum += red[t * P + e];
+    part[(size_t)rb * B + c] = sum;
   }
 }
 
@@ -1662,6 +1723,31 @@ int kpm_step_blk(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
   }
   return nb;
 }
+template <typename T>
+int kpm_step_bal(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1, const T *v0,
+                 double ctr, double scal, double *part, hipStream_t s) {
+  const int lv = bal_level((int)sizeof(T), B);
+  if (lv > 4 || !A.rowblk_q[lv] || A.nrowblk_q[lv] == 0) return 0;
+  const T *a = std::is_same<T, double>::value ? (const T *)A.a64 : (const T *)A.a32;
+  switch (B) {
+#define CASE(BB)                                                                          \
+  case BB:                                                                                \
+    hipLaunchKernelGGL((k_kpm_bal<T, BB>), dim3(A.nrowblk_q[lv]), dim3(kThreads), 0, s,   \
+                       A.rowblk_q[lv], A.ia, A.ja, a, vk, vkm1, vkp1, v0, (T)ctr, (T)scal, \
+                       part);                                                             \
+    break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32)
+#undef CASE
+    default:
+      return 0;
+  }
+  return A.nrowblk_q[lv];
+}
+template int kpm_step_bal<double>(const DevCsr &, int, const double *, const double *, double *,
+                                  const double *, double, double, double *, hipStream_t);
+template int kpm_step_bal<float>(const DevCsr &, int, const float *, const float *, float *,
+                                 const float *, double, double, double *, hipStream_t);
+
 template int kpm_step_blk<double>(const DevCsr &, int, const double *,
                                   const double *, double *, const double *,
                                   double, double, double *, hipStream_t);

@@ -104,6 +104,12 @@ int kpm_step_blk(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
                  const T *v0, double ctr, double scal, double *part,
                  hipStream_t s);
 int kpm_part_blocks(int n, int B);
+// The same step with the nnz-balanced products (uneven rows, B <= 32):
+// partials per row block of the balanced partition, returns their number
+// (0 if B has no partition).
+template <typename T>
+int kpm_step_bal(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1, const T *v0,
+                 double ctr, double scal, double *part, hipStream_t s);
 
 // ---- dense / vector kernels (FP64 unless stated) --------------------------
 // out[c] = sum_b part[b*ncol + c], b < nblk (fixed order); out may be strided.

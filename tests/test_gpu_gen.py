@@ -222,3 +222,26 @@ def test_gen_lap_flow_exact_counts(lib, bsol):
             assert np.all(np.asarray(s["res"]) < 1e-8)
         total += s["nev"]
     assert total == np.sum((lam >= r["sli"][0]) & (lam <= r["sli"][-1])) > 20
+
+
+@pytest.mark.parametrize("bsol", ["tridiag", "chebiter"])
+def test_genlap_c_driver(lib, bsol):
+    """pevsl_amd/genlap_plan (C, TESTS/Gen_Lap/LapPLanN.c defaults) finds the
+    dense pencil count in [1.5, 2.5]; "tridiag" is a host direct solver
+    through pEVSL_SetBSol, the place of the reference's MUMPS solve."""
+    import os
+    import re
+    import subprocess
+    banner("genlap_plan C driver, B-solve %s" % bsol)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "pevsl_amd",
+                       "genlap_plan")
+    out = subprocess.run([exe, "-bsol", bsol], capture_output=True, text=True, timeout=240)
+    print(out.stdout[-2000:], flush=True)
+    assert out.returncode == 0, out.stderr
+    total = int(re.search(r"total (\d+) eigenvalues", out.stdout).group(1))
+    A = dense(*driver.laplacian3d(8, 8, 8), 512)
+    B = dense(*driver.chain_laplacian(512), 512)
+    lam = sla.eigh(A, B, eigvals_only=True)
+    assert total == np.sum((lam >= 1.5) & (lam <= 2.5))
+    for m in re.finditer(r"max res ([0-9.e+-]+)", out.stdout):
+        assert float(m.group(1)) < 1e-8

@@ -459,6 +459,38 @@ def test_lan_trbounds_100cube_closed_form(lib):
     assert lo <= t + 1e-12 and hi >= tmax - 1e-12
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_kpmdos_uneven_rows_against_oracle(lib, prec):
+    """KPM on the power-law n=3000 matrix (rows up to 1,041 entries): the
+    nnz-balanced block step (blocks of 32 vectors; rows over 128 (FP64) /
+    256 (FP32) entries take the whole-workgroup path) against the oracle on
+    the same random stream.  FP64: relative 1e-12 (dot order differs);
+    FP32: absolute 2e-5 on mu."""
+    banner("Kpmdos power-law n=3000 Mdeg=100 nvec=64 prec=%d (balanced step)" % prec)
+    ia, ja, a = driver.powerlaw_sym(3000, seed=11)
+    M = driver.Matrix(lib, ia, ja, a)
+    lo, hi = -8.0, 70.0  # spectrum [-6.71, 62.5]
+    intv = [10.0, 14.0, lo, hi]
+    try:
+        inst = driver.Instance(lib, M, prec=prec)
+        try:
+            rc, mu, ecnt = inst.kpmdos(100, 1, 64, intv)
+        finally:
+            inst.close()
+    finally:
+        M.free()
+    assert rc == 0
+    orc.srand(0)
+    rc, omu, oecnt = orc.kpmdos(orc.Csr(ia, ja, a), 100, 1, 64, intv)
+    assert rc == 0
+    if prec == 0:
+        np.testing.assert_allclose(mu, omu, rtol=1e-12, atol=1e-15)
+        assert abs(ecnt - oecnt) < 1e-8
+    else:
+        np.testing.assert_allclose(mu, omu, rtol=0, atol=2e-5)
+        assert abs(ecnt - oecnt) < 0.05
+
+
 @pytest.mark.parametrize("prec,basis,rowk", [
     (capi.PEVSL_PREC_FP64, capi.PEVSL_PREC_FP64, -1),
     (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, -1),
