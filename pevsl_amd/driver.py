@@ -411,7 +411,7 @@ def cheblan_batch(inst, intvs, maxits, tol, vinit, pols, keep_vectors=False,
 def irregular_slicing_run(n, nslices=16, seed=4, frac=0.05, center=0.5, nvec=128, Mdeg=300,
                           prec=capi.PEVSL_PREC_FP64, basis_prec=capi.PEVSL_PREC_FP64,
                           batch=True, wave=None, slices_only=None, log=None, verbose=False,
-                          tune=True, block_rowkernel=-1):
+                          tune=True, block_rowkernel=-1, mlp=None, judge_tol=0.15):
     """The irregular-sparsity workload: powerlaw_sym(n) (mean row length 16
     before symmetrization, max 2000, exponent 2.2), bounds by
     pEVSL_LanTrbounds (lanm 50, maxit 200, tol 1e-8, as TESTS/MM/MMPLanN.c:111),
@@ -419,7 +419,10 @@ def irregular_slicing_run(n, nslices=16, seed=4, frac=0.05, center=0.5, nvec=128
     (0.5: the middle of the spectrum), KPM (Mdeg, nvec) slicing
     into `nslices`, FindPol and (batched) ChebLanNr as in lap_slicing_run.
     block_rowkernel selects the block SpMM (pEVSL_SetBlockKernel; an
-    autotune run overrides it).  Returns the DOS, slices and per-slice results
+    autotune run overrides it).  With `mlp` (a weights path) the slices
+    are the learned slicer's (cheap KPM + network), judged by the full KPM
+    run: when the full-KPM counts of the proposed slices are more than
+    judge_tol above their mean, the KPM slices are used instead.  Returns the DOS, slices and per-slice results
     (no closed form: callers check counts against their own reference)."""
     lib = capi.load()
     t0 = time.time()
@@ -446,6 +449,19 @@ def irregular_slicing_run(n, nslices=16, seed=4, frac=0.05, center=0.5, nvec=128
         rc, sli = spslicer_kpm(lib, mu, xintv, nslices, max(int(10 * ecount), 2 * nslices + 1))
         if rc:
             raise RuntimeError("spslicer error %d" % rc)
+        if mlp is not None:
+            from . import slicer as slc
+            net = slc.Mlp.load(mlp, lib)
+            try:
+                msli, mcount = slc.spslicer_mlp(inst, net, xintv, nslices)
+            finally:
+                net.free()
+            cnt, imb = slc.judge(lib, mu, xintv, n, msli)
+            kcnt, kimb = slc.judge(lib, mu, xintv, n, sli)
+            out["judge"] = {"counts": cnt.tolist(), "imbalance": imb, "kpm_imbalance": kimb,
+                            "ecount_mlp": mcount, "fallback": bool(imb > judge_tol)}
+            if imb <= judge_tol:
+                sli = msli
         out.update(bounds=[lmin, lmax], intv=[a, b], ecount=ecount, sli=sli, mu=mu,
                    t_dos=time.time() - t1)
         ev_int = int(1 + ecount / float(nslices))

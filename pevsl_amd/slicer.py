@@ -17,6 +17,10 @@ training recipe:
   (normalised) and log(fraction of the spectrum inside);
 * network 138 -> 3 x 256 (GELU) -> 513, trained with Adam on the GPU.
 
+An irregular family can be added (--family powerlaw): driver.powerlaw_sym
+matrices with n log-uniform in [1e5, 5e5], framed by pEVSL_LanTrbounds, half
+of their intervals a 2-10% share centred in the middle of the frame.
+
 Command line (writes the weights and a JSON record):
     python -m pevsl_amd.slicer --out pevsl_amd/weights/slicer_lap.bin
 """
@@ -260,18 +264,43 @@ def draw_interval(rng, lmin, lmax):
     return a, a + wd
 
 
-def gen_moments(lib, nmat, seed, log=print, full=(300, 64), cheap=None):
-    """Per matrix: full and cheap KPM moments from this library's Kpmdos."""
+def draw_interval_irregular(rng, lmin, lmax):
+    """Half the intervals of the irregular family as draw_interval, half a
+    2-10% share of the frame centred in its middle 40% (the regime of the
+    irregular workload: a 5% share at the centre)."""
+    if rng.random() < 0.5:
+        return draw_interval(rng, lmin, lmax)
+    L = lmax - lmin
+    c = lmin + L * rng.uniform(0.3, 0.7)
+    h = 0.5 * L * rng.uniform(0.02, 0.1)
+    return c - h, c + h
+
+
+def gen_moments(lib, nmat, seed, log=print, full=(300, 64), cheap=None, family="lap",
+                nmin=100000, nmax=500000):
+    """Per matrix: full and cheap KPM moments from this library's Kpmdos.
+    family "lap": Laplacians with potentials framed by Gershgorin bounds;
+    "powerlaw": driver.powerlaw_sym matrices with n log-uniform in
+    [nmin, nmax], framed by pEVSL_LanTrbounds(50, 200, 1e-8) (marked w = -1)."""
     cheap = cheap or (capi.PEVSL_SLICER_MDEG, capi.PEVSL_SLICER_NVEC)
     rng = np.random.default_rng(seed)
     mats = []
     t0 = time.time()
     for j in range(nmat):
-        nx, ny, nz, w = draw_matrix(rng)
-        ia, ja, a, (lmin, lmax) = lap_potential(nx, ny, nz, w, seed * 100003 + j)
+        if family == "powerlaw":
+            n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+            ia, ja, a = driver.powerlaw_sym(n, seed=seed * 100003 + j)
+            nx, ny, nz, w = n, 1, 1, -1.0
+        else:
+            nx, ny, nz, w = draw_matrix(rng)
+            ia, ja, a, (lmin, lmax) = lap_potential(nx, ny, nz, w, seed * 100003 + j)
         M = driver.Matrix(lib, ia, ja, a)
         inst = driver.Instance(lib, M)
         try:
+            if family == "powerlaw":
+                rc, lmin, lmax = inst.lan_trbounds(50, 200, 1e-8)
+                if rc:
+                    raise RuntimeError("LanTrbounds failed")
             frame = [lmin, lmax, lmin, lmax]
             rc, mu_full, _ = inst.kpmdos(full[0], 1, full[1], frame)
             rc2, mu_cheap, _ = inst.kpmdos(cheap[0], 1, cheap[1], frame)
@@ -282,7 +311,7 @@ def gen_moments(lib, nmat, seed, log=print, full=(300, 64), cheap=None):
             M.free()
         mats.append({"dims": (nx, ny, nz), "w": w, "frame": (lmin, lmax), "n": nx * ny * nz,
                      "mu_full": mu_full, "mu_cheap": mu_cheap})
-        if (j + 1) % 100 == 0 or j + 1 == nmat:
+        if (j + 1) % (10 if family == "powerlaw" else 100) == 0 or j + 1 == nmat:
             log("moments: %d/%d matrices, %.1f s" % (j + 1, nmat, time.time() - t0))
             sys.stdout.flush()
     return mats
@@ -302,15 +331,16 @@ def load_moments(path):
              "mu_cheap": z["mu_cheap"][i]} for i in range(len(z["n"]))]
 
 
-def make_pairs(lib, mats, nint, seed, log=print):
+def make_pairs(lib, mats, nint, seed, log=print, nint_irregular=None):
     """Training pairs (features, prior, target) for `nint` random intervals
-    per matrix; targets are full-KPM bin masses (see module doc)."""
+    per matrix (`nint_irregular` for the irregular family); targets are
+    full-KPM bin masses (see module doc)."""
     rng = np.random.default_rng(seed + 7)
     X, Q, T, meta = [], [], [], []
     for j, m in enumerate(mats):
         lmin, lmax = m["frame"]
-        for _ in range(nint):
-            lo, hi = draw_interval(rng, lmin, lmax)
+        for _ in range(nint_irregular or nint if m["w"] < 0 else nint):
+            lo, hi = (draw_interval_irregular if m["w"] < 0 else draw_interval)(rng, lmin, lmax)
             iv = [lo, hi, lmin, lmax]
             ms, fr = bin_masses(lib, m["mu_full"], iv)
             ms = np.maximum(ms, 0.0)
@@ -379,6 +409,14 @@ def main(argv=None):
                          "they all exist, else the first is generated and written")
     ap.add_argument("--gen-only", action="store_true",
                     help="only generate and write the moments")
+    ap.add_argument("--family", default="lap", choices=("lap", "powerlaw"),
+                    help="matrix family of generated moments")
+    ap.add_argument("--nint-irregular", type=int, default=None,
+                    help="intervals per matrix of the irregular family (default --nint)")
+    ap.add_argument("--split", default="tail", choices=("tail", "mod10"),
+                    help="held-out matrices: the last --holdout share, or every 10th")
+    ap.add_argument("--nmin", type=int, default=100000)
+    ap.add_argument("--nmax", type=int, default=500000)
     args = ap.parse_args(argv)
     lib = capi.load()
     t0 = time.time()
@@ -386,17 +424,22 @@ def main(argv=None):
     if files and all(os.path.exists(f) for f in files) and not args.gen_only:
         mats = [m for f in files for m in load_moments(f)]
     else:
-        mats = gen_moments(lib, args.nmat, args.seed)
+        mats = gen_moments(lib, args.nmat, args.seed, family=args.family, nmin=args.nmin,
+                           nmax=args.nmax)
         if files:
             save_moments(files[0], mats)
         if args.gen_only:
             return 0
-    X, Q, T, meta = make_pairs(lib, mats, args.nint, args.seed)
+    X, Q, T, meta = make_pairs(lib, mats, args.nint, args.seed,
+                               nint_irregular=args.nint_irregular)
     t_data = time.time() - t0
     nmat = len(mats)
     idx = meta[:, 0].astype(int)
-    nval = max(1, int(round(args.holdout * nmat)))
-    tr = idx < nmat - nval
+    if args.split == "mod10":
+        tr = idx % 10 != 9
+    else:
+        nval = max(1, int(round(args.holdout * nmat)))
+        tr = idx < nmat - nval
     va = ~tr
     mlp = Mlp(lib, seed=args.seed)
     t1 = time.time()
@@ -410,11 +453,17 @@ def main(argv=None):
         ev["heldout_%d_slices" % ns] = evaluate(lib, mlp, X[va], Q[va], T[va], meta[va], ns)
     ev["train_10_slices"] = evaluate(lib, mlp, X[tr][:3000], Q[tr][:3000], T[tr][:3000],
                                      meta[tr][:3000], 10)
+    irr = meta[:, 4].astype(float) < 0
+    if np.any(va & irr):
+        for ns in (10, 16):
+            ev["heldout_irregular_%d_slices" % ns] = evaluate(
+                lib, mlp, X[va & irr], Q[va & irr], T[va & irr], meta[va & irr], ns)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     mlp.save(args.out)
     rec = {"weights": os.path.basename(args.out), "arch": [NIN, 256, 3, NOUT, NBINS],
-           "recipe": {k: getattr(args, k) for k in ("nmat", "nint", "epochs", "batch", "lr",
-                                                     "seed", "holdout")},
+           "recipe": {k: getattr(args, k) for k in ("nmat", "nint", "nint_irregular", "epochs",
+                                                     "batch", "lr", "seed", "holdout", "split",
+                                                     "moments")},
            "matrices": nmat, "samples": {"train": int(tr.sum()), "heldout": int(va.sum())},
            "seconds": {"dataset": round(t_data, 1), "train": round(t_train, 1)},
            "loss": {"first_epoch": float(losses[0]), "last_epoch": float(losses[-1]),
