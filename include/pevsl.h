@@ -84,8 +84,7 @@ void pEVSL_LSPolFree(void *vdata);
  * (every product then crosses PCIe).  SetBSol / SetLTSol install the solves
  * of a generalized problem (host callbacks, or pEVSL_LSPolSol, which runs on
  * the device); SetGenEig switches Kpmdos, LanTrbounds, ChebAv and ChebLanNr
- * to A x = lambda B x (SRC/pevsl.c:123-210).  The generalized LanDosG
- * returns PEVSL_NOT_IMPLEMENT. */
+ * to A x = lambda B x (SRC/pevsl.c:123-210), and LanDosG as well. */
 int pEVSL_Start(MPI_Comm comm, pevsl_Data **data);
 int pEVSL_Finish(pevsl_Data *pevsl_data);
 int pEVSL_SetProbSizes(pevsl_Data *pevsl_data, int N, int n, int nfirst);

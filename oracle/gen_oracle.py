@@ -141,3 +141,65 @@ def chebiter_solv2(B, b, lb, ub, deg):
         d = rho1 * rho * d + (2.0 * rho1 / delta) * r
         rho = rho1
     return x
+
+
+def landos_gen(A, B, solveB, solveLT, vecs, msteps, npts, intv, sigma_mult=1.0):
+    """Generalized Lanczos DOS (SRC/landos.c:40-314 with ifGenEv): per start
+    vector r (rows of vecs), v = L^{-T} r normalised in the B-norm, msteps
+    Lanczos steps with Z = B V, CGS_DGKS2 (two passes) and B-solves; the DOS
+    is the sum of Gaussians of width sigma at the Ritz values weighted by the
+    squared first eigenvector components.  Returns (xdos, ydos) before the
+    Simpson integration; a breakdown before the last step raises (the
+    library draws a fresh random vector there)."""
+    n = A.shape[0]
+    maxit = min(n, msteps)
+    lm, lM = intv[2], intv[3]
+    aa, bb = max(intv[0], intv[2]), min(intv[1], intv[3])
+    M = min(msteps, 30)
+    H = (lM - lm) / (M - 1)
+    sigma = H / math.sqrt(8 * math.log(1.25)) * sigma_mult
+    sigma2 = 2 * sigma * sigma
+    width = sigma * math.sqrt(-2.0 * math.log(1e-08))
+    h = (bb - aa) / (npts - 1) if npts > 1 else 0.0  # linspace of SRC/vector.c:15-25
+    xdos = np.array([aa + i * h for i in range(npts)])
+    xdos[0], xdos[-1] = aa, bb
+    y = np.zeros(npts)
+    for r in vecs:
+        v = solveLT(np.asarray(r, dtype=np.float64))
+        z = B @ v
+        t = 1.0 / math.sqrt(v @ z)
+        V, Z = [v * t], [z * t]
+        dT, eT = np.zeros(maxit), np.zeros(maxit)
+        beta, wn, nwn = 0.0, 0.0, 0
+        for k in range(maxit):
+            znew = A @ V[k]
+            if k > 0:
+                znew = znew - beta * Z[k - 1]
+            alpha = V[k] @ znew
+            dT[k] = alpha
+            wn += abs(alpha)
+            znew = znew - alpha * Z[k]
+            Vm, Zm = np.array(V).T, np.array(Z).T
+            for _ in range(2):
+                znew = znew - Zm @ (Vm.T @ znew)
+            vnew = solveB(znew)
+            beta = math.sqrt(vnew @ znew)
+            wn += 2.0 * beta
+            nwn += 3
+            if beta * nwn < 1e-14 * wn:
+                if k < maxit - 1:
+                    raise RuntimeError("breakdown at step %d" % k)
+                beta = 0.0
+                vnew, znew = np.zeros(n), np.zeros(n)
+            else:
+                vnew, znew = vnew / beta, znew / beta
+            eT[k] = beta
+            V.append(vnew)
+            Z.append(znew)
+        th, S = np.linalg.eigh(np.diag(dT) + np.diag(eT[:-1], 1) + np.diag(eT[:-1], -1))
+        g2 = S[0, :] ** 2
+        for i in range(maxit):
+            sel = np.abs(xdos - th[i]) < width
+            y[sel] += g2[i] * np.exp(-((xdos[sel] - th[i]) ** 2) / sigma2)
+    y *= 1.0 / (len(vecs) * math.sqrt(sigma2 * math.pi))
+    return xdos, y

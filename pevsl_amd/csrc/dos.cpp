@@ -316,14 +316,17 @@ int pEVSL_LanDosG(pevsl_Data *pevsl, const int nvec, int msteps, int npts, doubl
   (void)groupid;
   (void)gl_comm;
   if (ngroups > 1) return PEVSL_NOT_IMPLEMENT;
-  // the generalized Lanczos DOS (SRC/landos.c ifGenEv branches) is not built:
-  // generalized problems slice with pEVSL_Kpmdos
-  if (pevsl->ifGenEv) return PEVSL_NOT_IMPLEMENT;
   Ctx &c = ctx_of(pevsl);
   const double tm = wtime();
   const int n = pevsl->n;
   const int maxit = std::min(pevsl->N, msteps);
   const hipStream_t s = c.stream;
+  std::vector<double> dT((size_t)nvec * maxit), eT((size_t)nvec * maxit);
+  if (pevsl->ifGenEv) {
+    // generalized problem: one random vector after the other (SRC/landos.c
+    // ifGenEv branches)
+    landos_gen(c, pevsl, nvec, maxit, dT, eT);
+  } else {
   // bases: one per random vector, maxit+1 columns each (single chunk)
   LanBatch L;
   L.n = n;
@@ -371,7 +374,7 @@ int pEVSL_LanDosG(pevsl_Data *pevsl, const int nvec, int msteps, int npts, doubl
       scale_vec(n, vt[(size_t)v], nullptr, 1.0 / sqrt(nr[0]), s);
     }
   }
-  std::vector<double> dT((size_t)nvec * maxit), eT((size_t)nvec * maxit), wn((size_t)nvec, 0.0);
+  std::vector<double> wn((size_t)nvec, 0.0);
   std::vector<int> nwn((size_t)nvec, 0);
   std::vector<double> ha((size_t)2 * nvec), betas((size_t)nvec);
   const int lanes = c.lanes ? c.lanes : auto_lanes((double)c.A->nnz / std::max(1, n));
@@ -426,6 +429,7 @@ int pEVSL_LanDosG(pevsl_Data *pevsl, const int nvec, int msteps, int npts, doubl
       betas[(size_t)v] = 0.0;
     }
     for (int v = 0; v < nvec; v++) eT[(size_t)v * maxit + k] = betas[(size_t)v];
+  }
   }
   // Ritz values and first components of the eigenvectors of each T
   DBuf<double> D, E, C;

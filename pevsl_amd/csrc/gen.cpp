@@ -266,6 +266,60 @@ void kpm_moments_gen(Ctx &c, pevsl_Data *p, int Mdeg, int nvec, double ctr, doub
   }
 }
 
+// Tridiagonal matrices of the generalized Lanczos DOS (SRC/landos.c:129-275,
+// ifGenEv): per random vector, v = L^{-T} r / |.|_B, maxit Lanczos steps with
+// bases V and Z = B V, CGS_DGKS2 and B-solves; dT/eT[m * maxit + k].
+void landos_gen(Ctx &c, pevsl_Data *p, int nvec, int maxit, std::vector<double> &dT,
+                std::vector<double> &eT) {
+  const int n = p->n;
+  const hipStream_t s = c.stream;
+  const size_t ld = ((size_t)n + 63) / 64 * 64;
+  DBuf<double> Vb, Zb, rb;
+  double *V = Vb.ensure(ld * (size_t)(maxit + 1)), *Z = Zb.ensure(ld * (size_t)(maxit + 1));
+  double *r = rb.ensure(ld);
+  auto vcol = [&](int j) { return V + (size_t)j * ld; };
+  auto zcol = [&](int j) { return Z + (size_t)j * ld; };
+  for (int m = 0; m < nvec; m++) {
+    rand_vectors(c, n, 1, r, ld);
+    solve_LT(c, p, r, vcol(0));
+    apply_B(c, p, vcol(0), zcol(0));
+    double t = 1.0 / sqrt(ddot(c, n, vcol(0), zcol(0)));
+    scale_vec(n, zcol(0), nullptr, t, s);
+    scale_vec(n, vcol(0), nullptr, t, s);
+    double wn = 0.0, beta = 0.0;
+    int nwn = 0;
+    for (int k = 0; k < maxit; k++) {
+      double *v = vcol(k), *z = zcol(k), *vnew = vcol(k + 1), *znew = zcol(k + 1);
+      apply_A(c, v, znew);
+      if (k > 0) axpy_vec(n, -beta, zcol(k - 1), znew, s);
+      const double alpha = ddot(c, n, v, znew);
+      dT[(size_t)m * maxit + k] = alpha;
+      wn += fabs(alpha);
+      axpy_vec(n, -alpha, z, znew, s);
+      cgs2(c, n, k + 1, NGS_MAX, V, Z, ld, znew);
+      solve_B(c, p, znew, vnew);
+      beta = sqrt(ddot(c, n, vnew, znew));
+      wn += 2.0 * beta;
+      nwn += 3;
+      if (beta * nwn < orthTol * wn) {
+        rand_vectors(c, n, 1, vnew, ld);
+        cgs2(c, n, k + 1, NGS_MAX, Z, V, ld, vnew);
+        apply_B(c, p, vnew, znew);
+        beta = sqrt(ddot(c, n, vnew, znew));
+        t = 1.0 / beta;
+        scale_vec(n, vnew, nullptr, t, s);
+        scale_vec(n, znew, nullptr, t, s);
+        beta = 0.0;
+      } else {
+        t = 1.0 / beta;
+        scale_vec(n, vnew, nullptr, t, s);
+        scale_vec(n, znew, nullptr, t, s);
+      }
+      eT[(size_t)m * maxit + k] = beta;
+    }
+  }
+}
+
 // Non-restarted filtered Lanczos for A x = lambda B x (SRC/cheblanNr.c:
 // 54-448, ifGenEv branches): bases V and Z = B V in HBM, the filter
 // p(A B^{-1}) applied to z, CGS_DGKS2 against (V, Z), B-inner products.

@@ -173,6 +173,9 @@ void kpm_moments_gen(Ctx &c, pevsl_Data *p, int Mdeg, int nvec, double ctr, doub
 bool chebiter_device_solve(SVFunc f, void *data, const double *dx, double *dy, double *wk,
                            hipStream_t s);
 int chebiter_size(void *data);
+// tridiagonal matrices of the generalized Lanczos DOS, dT/eT[m*maxit + k]
+void landos_gen(Ctx &c, pevsl_Data *p, int nvec, int maxit, std::vector<double> &dT,
+                std::vector<double> &eT);
 // generalized ChebLanNr for one slice (outputs as pEVSL_ChebLanNr)
 int cheblan_gen(Ctx &c, pevsl_Data *p, double *intv, int maxit, double tol,
                 pevsl_Parvec *vinit, pevsl_Polparams *pol, int *nevOut, double **lamo,

@@ -148,3 +148,30 @@ def test_chebiter_both_forms_converge_to_the_solve():
         x = f(B, b, ev[0], ev[-1], 200)
         assert np.linalg.norm(x - ex) < 1e-12 * np.linalg.norm(ex)
     assert go.chebiter_solv1(B, b, ev[0], ev[-1], 0) is None
+
+
+def test_landos_gen_full_lanczos_is_the_spectral_measure():
+    """With msteps = n the Lanczos quadrature of the pencil is exact: nodes =
+    eigenvalues of (A, B), weights = (x_j^T B v)^2 for the B-normalised start
+    v = L^{-T} r; the DOS is then the sum of the weighted Gaussians."""
+    A, B = _pencil(20, 21)
+    lam, X = sla.eigh(A, B)
+    ev, U = np.linalg.eigh(B)
+    Bmh = U @ np.diag(ev ** -0.5) @ U.T
+    r = np.random.default_rng(22).uniform(-1, 1, (2, 20))
+    intv = [lam[0], lam[-1], lam[0] - 0.1, lam[-1] + 0.1]
+    xd, yd = go.landos_gen(A, B, lambda q: np.linalg.solve(B, q), lambda q: Bmh @ q, r, 20,
+                           300, intv)
+    M = 20
+    sigma = (intv[3] - intv[2]) / (M - 1) / math.sqrt(8 * math.log(1.25))
+    ref = np.zeros_like(xd)
+    for q in r:
+        v = Bmh @ q
+        v = v / math.sqrt(v @ B @ v)
+        w = (X.T @ (B @ v)) ** 2
+        assert abs(w.sum() - 1) < 1e-12
+        for lj, wj in zip(lam, w):
+            sel = np.abs(xd - lj) < sigma * math.sqrt(-2 * math.log(1e-8))
+            ref[sel] += wj * np.exp(-((xd[sel] - lj) ** 2) / (2 * sigma * sigma))
+    ref /= 2 * math.sqrt(2 * sigma * sigma * math.pi)
+    assert np.max(np.abs(yd - ref)) < 1e-10 * np.max(ref)

@@ -156,6 +156,30 @@ def test_kpm_generalized_matches_oracle(lib):
         g.close()
 
 
+def test_landos_generalized_matches_oracle(lib):
+    """Generalized LanDosG (8^3 Gen_Lap pair, exact B-solve, LSPol L^T
+    solve) against the dense oracle on the same random stream: the DOS curve
+    to 1e-9 relative, the count within 30% of the dense pencil count."""
+    banner("generalized LanDosG 8^3, 8 vectors, 30 steps")
+    g = Gen(lib, 8, 8, 8, "host")
+    try:
+        lmin, lmax = g.lam[0] * 0.99, g.lam[-1] * 1.01
+        xintv = [1.5, 2.5, lmin, lmax]
+        rc, xd, yd, neig = g.inst.landos(8, 30, 200, xintv)
+        assert rc == 0
+        orc.srand(0)
+        vecs = [orc.rand_vec(g.n) for _ in range(8)]
+        oxd, oyd = go.landos_gen(g.A, g.B, g.solveB, g.solveLT, vecs, 30, 200, xintv)
+        np.testing.assert_allclose(xd, oxd, rtol=0, atol=1e-15)
+        assert np.max(np.abs(yd - oyd)) < 1e-9 * np.max(np.abs(oyd))
+        oneig = orc.simpson(oxd, oyd)[-1] * g.n
+        assert abs(neig - oneig) < 1e-8 * abs(oneig)
+        exact = np.sum((g.lam >= 1.5) & (g.lam <= 2.5))
+        assert abs(neig - exact) < 0.3 * exact
+    finally:
+        g.close()
+
+
 @pytest.mark.parametrize("bsol", ["host", "lspol"])
 def test_lantrbounds_generalized(lib, bsol):
     banner("generalized LanTrbounds 8^3, B-solve %s" % bsol)
