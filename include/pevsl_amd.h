@@ -194,6 +194,12 @@ typedef struct {
 } pevsl_TuneResult;
 int pEVSL_Autotune(pevsl_Data *pevsl, int reps, pevsl_TuneResult *res);
 
+/* Times `reps` back-to-back filter steps of width B (power of two <= 32)
+ * with the block kernel a batched solve of the instance would use
+ * (pEVSL_SetBlockKernel; Clenshaw forms need the FP32 filter): seconds per
+ * step and its algorithmic bytes.  -1 on invalid arguments. */
+int pEVSL_BenchBlock(pevsl_Data *pevsl, int B, int reps, double *sec, double *bytes);
+
 /* Times `reps` launches of a plain copy kernel over n doubles (read n,
  * write n; 16-byte accesses) on the current device: seconds per launch and
  * its bytes (16 n).  A bandwidth reference for the other kernels and the

@@ -226,6 +226,7 @@ SIGNATURES = {
     "pEVSL_SetKeepVectors": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetBlockKernel": (C.c_int, [DataP, C.c_int]),
     "pEVSL_RandSeed": (None, [C.c_uint]),
+    "pEVSL_BenchBlock": (C.c_int, [DataP, C.c_int, C.c_int, c_double_p, c_double_p]),
     "pEVSL_ChebIterSetup": (C.c_int, [C.c_double, C.c_double, C.c_int, C.POINTER(Parcsr),
                                       C.POINTER(C.c_void_p)]),
     "pEVSL_ChebIterSolv1": (None, [c_double_p, c_double_p, C.c_void_p]),

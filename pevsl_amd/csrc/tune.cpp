@@ -176,6 +176,25 @@ int pEVSL_Autotune(pevsl_Data *p, int reps, pevsl_TuneResult *res) {
   return 0;
 }
 
+int pEVSL_BenchBlock(pevsl_Data *p, int B, int reps, double *sec, double *bytes) {
+  Ctx &c = ctx_of(p);
+  if (!c.A || reps < 1 || !sec || !bytes || B < 1 || B > 32 || (B & (B - 1))) return -1;
+  const bool f32 = c.filter_prec == PEVSL_PREC_FP32;
+  const int elem = f32 ? 4 : 8;
+  // the kernel a batched solve of width B uses (see BatchSolver::filter_block)
+  int kind = c.blk_rowkernel;
+  if (kind < 0) kind = (f32 ? 2 : 1) + (c.A->max_row > kUnevenRow ? 2 : 0);
+  if (!f32 && (kind == 2 || kind == 4)) return -1;  // Clenshaw forms: FP32 only
+  DBuf<double> mus;
+  mus.ensure(3 * 64);
+  std::vector<double> mh(3 * 64, 0.01);
+  PV_HIP(hipMemcpy(mus.p, mh.data(), sizeof(double) * mh.size(), hipMemcpyHostToDevice));
+  *sec = f32 ? time_blk<float>(c, B, reps, mus, kind) : time_blk<double>(c, B, reps, mus, kind);
+  *bytes = (kind == 2 || kind == 4) ? step_bytes(*c.A, elem, B) - (double)c.A->n * B * elem
+                                    : step_bytes(*c.A, elem, B);
+  return 0;
+}
+
 int pEVSL_BenchCopy(size_t n, int reps, double *sec, double *bytes) {
   if (n < 2 || reps < 1 || !sec || !bytes) return -1;
   n &= ~(size_t)1;

@@ -14,7 +14,13 @@ this probe launches each hot kernel a few times on the 100^3 Laplacian
              including the width-8 Clenshaw block SpMM) and
              pEVSL_BenchReorth, k = 1024, FP32 basis.
 
-usage: python -m pevsl_amd.profile_probe
+  irregular  (--irregular) the power-law matrix of the irregular workload
+             (n = 2e6, nnz = 6.6e7): the FP32 Clenshaw filter step at block
+             width 16 with the row-per-thread kernel and with the
+             nnz-balanced kernel (pEVSL_SetBlockKernel 2 and 4), 5 launches
+             each after one warm-up (pEVSL_BenchBlock).
+
+usage: python -m pevsl_amd.profile_probe [--irregular]
 """
 import ctypes as C
 import sys
@@ -24,7 +30,37 @@ import numpy as np
 from . import capi, driver
 
 
+def irregular():
+    """Row kernel vs nnz-balanced kernel on the irregular workload's matrix:
+    pEVSL_BenchBlock times back-to-back FP32 Clenshaw steps of width 16 with
+    the instance's block kernel."""
+    lib = capi.load()
+    sec, byt = C.c_double(), C.c_double()
+    lib.pEVSL_BenchCopy(1 << 26, 5, C.byref(sec), C.byref(byt))
+    print("[probe] copy %.3f ms" % (sec.value * 1e3), flush=True)
+    print("[probe] generating the power-law matrix n=2e6", flush=True)
+    ia, ja, a = driver.powerlaw_sym(2000000, seed=4)
+    M = driver.Matrix(lib, ia, ja, a)
+    try:
+        for kind in (2, 4):
+            inst = driver.Instance(lib, M, prec=capi.PEVSL_PREC_FP32, block_rowkernel=kind)
+            try:
+                rc = lib.pEVSL_BenchBlock(inst.p, 16, 5, C.byref(sec), C.byref(byt))
+                assert rc == 0
+                print("[probe] block kernel %d, width 16: %.1f us per step, %.1f us per column,"
+                      " %.2f TB/s algorithmic" % (kind, sec.value * 1e6, sec.value * 1e6 / 16,
+                                                  byt.value / sec.value / 1e12), flush=True)
+            finally:
+                inst.close()
+    finally:
+        M.free()
+    print("[probe] done", flush=True)
+    return 0
+
+
 def main():
+    if "--irregular" in sys.argv[1:]:
+        return irregular()
     lib = capi.load()
     print("[probe] copy", flush=True)
     sec, byt = C.c_double(), C.c_double()
