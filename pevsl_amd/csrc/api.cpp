@@ -205,6 +205,8 @@ int pEVSL_SetProbSizes(pevsl_Data *p, int N, int n, int nfirst) {
 
 int pEVSL_SetAParcsr(pevsl_Data *p, pevsl_Parcsr *A) {
   Ctx &c = ctx_of(p);
+  // the eigenvalue problem needs a square operator
+  if (A->nrow_global != A->ncol_global || A->nrow_local != A->ncol_local) return -1;
   p->N = A->ncol_global;
   p->n = A->ncol_local;
   p->nfirst = A->first_col;
@@ -485,14 +487,14 @@ int pEVSL_ParcsrLocalNnz(pevsl_Parcsr *A) {
 void pEVSL_ParcsrMatvec(pevsl_Parcsr *A, pevsl_Parvec *x, pevsl_Parvec *y) {
   const DevCsr *M = dev_matrix_of(A);
   if (!M) fatal("ParcsrMatvec: no device matrix", __FILE__, __LINE__);
-  const int n = A->nrow_local;
+  // y (nrow_local) = A x (ncol_local); A may be rectangular
+  const int n = A->nrow_local, m = A->ncol_local;
+  if (x->n_local != m || y->n_local != n) fatal("ParcsrMatvec: vector sizes", __FILE__, __LINE__);
   double *d = nullptr;
-  PV_HIP(hipMalloc(&d, sizeof(double) * 2 * (size_t)(n > 0 ? n : 1)));
-  PV_HIP(hipMemcpy(d, x->data, sizeof(double) * (size_t)n,
-                   hipMemcpyHostToDevice));
-  spmv_f64(*M, d, d + n, auto_lanes(n ? (double)M->nnz / n : 1.0), 0);
-  PV_HIP(hipMemcpy(y->data, d + n, sizeof(double) * (size_t)n,
-                   hipMemcpyDeviceToHost));
+  PV_HIP(hipMalloc(&d, sizeof(double) * ((size_t)n + (size_t)m + 1)));
+  PV_HIP(hipMemcpy(d, x->data, sizeof(double) * (size_t)m, hipMemcpyHostToDevice));
+  spmv_f64(*M, d, d + m, auto_lanes(n ? (double)M->nnz / n : 1.0), 0);
+  PV_HIP(hipMemcpy(y->data, d + m, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
   PV_HIP(hipFree(d));
 }
 

@@ -269,3 +269,50 @@ def test_genlap_c_driver(lib, bsol):
     assert total == np.sum((lam >= 1.5) & (lam <= 2.5))
     for m in re.finditer(r"max res ([0-9.e+-]+)", out.stdout):
         assert float(m.group(1)) < 1e-8
+
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_lapchebiter_flow(lib, kind):
+    """TESTS0/Chebiter/LapChebiter.c defaults: A = ParcsrLaplace(512, 1, 1),
+    bounds by LanTrbounds(50, 200, 1e-8), 10 Chebyshev-iteration steps; the
+    device solve equals the oracle's on the same b (1e-12), and for Saad's
+    form the residual equals the closed-form residual polynomial
+    T_10((theta - A)/delta)/T_10(theta/delta) b on the exact spectrum."""
+    banner("LapChebiter flow, type %d" % kind)
+    n, deg = 512, 10
+    ia, ja, a = driver.chain_laplacian(n)
+    M = driver.Matrix(lib, ia, ja, a)
+    As = sp.csr_matrix((a, ja, ia), shape=(n, n))
+    cd = C.c_void_p()
+    try:
+        inst = driver.Instance(lib, M)
+        try:
+            rc, lmin, lmax = inst.lan_trbounds(50, 200, 1e-8)
+            assert rc == 0
+        finally:
+            inst.close()
+        ex = 4 - 2 * np.cos(np.arange(1, n + 1) * np.pi / (n + 1))
+        assert lmin <= ex.min() + 1e-8 and lmax >= ex.max() - 1e-8
+        b = np.random.default_rng(5).uniform(-1, 1, n)
+        x = np.zeros(n)
+        assert lib.pEVSL_ChebIterSetup(lmin, lmax, deg, C.byref(M.A), C.byref(cd)) == 0
+        solv = lib.pEVSL_ChebIterSolv1 if kind == 1 else lib.pEVSL_ChebIterSolv2
+        solv(b.ctypes.data_as(capi.c_double_p), x.ctypes.data_as(capi.c_double_p), cd)
+        ref = (go.chebiter_solv1 if kind == 1 else go.chebiter_solv2)(As, b, lmin, lmax, deg)
+        assert np.linalg.norm(x - ref) < 1e-12 * np.linalg.norm(ref)
+        res = b - As @ x
+        if kind == 2:
+            U = np.sin(np.outer(np.arange(1, n + 1), np.arange(1, n + 1)) * np.pi / (n + 1))
+            U *= math.sqrt(2.0 / (n + 1))
+            theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+            tk = np.polynomial.chebyshev.chebval((theta - ex) / delta, [0] * deg + [1])
+            tk0 = np.polynomial.chebyshev.chebval(theta / delta, [0] * deg + [1])
+            rref = U @ ((tk / tk0) * (U.T @ b))
+            assert np.linalg.norm(res - rref) < 1e-12 * np.linalg.norm(b)
+        # contraction of 10 steps on kappa ~ 3: far below the start residual
+        assert np.linalg.norm(res) < 1e-4 * np.linalg.norm(b)
+    finally:
+        if cd:
+            lib.pEVSL_ChebIterFree(cd)
+        M.free()

@@ -554,3 +554,36 @@ def test_matrix_market_flow_stiff1(lib, a, b, ns):
             assert np.all(np.asarray(s["res"]) < 1e-8)
         total += s["nev"]
     assert total == np.sum((ev >= a) & (ev <= b))
+
+
+
+def test_rectangular_parcsr_matvec(lib):
+    """TESTS0/Matvec/test4.c on one rank: a rectangular random sparse matrix
+    (300 x 500), x_i = sin(i) (TESTS0/Matvec/test1.c:70-94), y = A x on the
+    GPU against scipy: relative 1e-14.  A square-only instance rejects it."""
+    banner("rectangular ParcsrMatvec 300 x 500")
+    import scipy.sparse as sp
+    R = sp.random(300, 500, density=0.03, random_state=np.random.default_rng(17), format="csr")
+    R.sort_indices()
+    ia, ja, a = R.indptr.astype(np.int32), R.indices.astype(np.int32), R.data.astype(np.float64)
+    csr = capi.Csr(300, 500, ia.ctypes.data_as(capi.c_int_p), ja.ctypes.data_as(capi.c_int_p),
+                   a.ctypes.data_as(capi.c_double_p))
+    A = capi.Parcsr()
+    assert lib.pEVSL_ParcsrCreate(300, 500, None, None, C.byref(A), capi.MPI_COMM_WORLD) == 0
+    assert lib.pEVSL_ParcsrSetup(C.byref(csr), C.byref(A)) == 0
+    try:
+        x = np.sin(np.arange(500, dtype=np.float64))
+        y = np.zeros(300)
+        X, Y = capi.Parvec(), capi.Parvec()
+        lib.pEVSL_ParvecCreateShell(500, 500, 0, 0, C.byref(X), x.ctypes.data_as(capi.c_double_p))
+        lib.pEVSL_ParvecCreateShell(300, 300, 0, 0, C.byref(Y), y.ctypes.data_as(capi.c_double_p))
+        lib.pEVSL_ParcsrMatvec(C.byref(A), C.byref(X), C.byref(Y))
+        ref = R @ x
+        assert np.linalg.norm(y - ref) <= 1e-14 * np.linalg.norm(ref)
+        inst = driver.Instance(lib)
+        try:
+            assert lib.pEVSL_SetAParcsr(inst.p, C.byref(A)) == -1
+        finally:
+            inst.close()
+    finally:
+        lib.pEVSL_ParcsrFree(C.byref(A))
