@@ -334,3 +334,39 @@ def test_landos_count_and_slices():
     assert abs(neig - exact) < 0.5 * exact
     sli = orc.spslicer_lan(xd, yd, 4)
     assert sli[0] == 0.6 and sli[-1] == 1.2 and np.all(np.diff(sli) > 0)
+
+
+def test_landos_full_lanczos_is_the_spectral_measure():
+    """With msteps = n the Lanczos quadrature is exact: the DOS curve of
+    orc_landos is the sum of Gaussians at the eigenvalues weighted by the
+    squared components (u_j . v)^2 of the normalised start vector (the libc
+    stream after srand(0)).  One start vector, so the breakdown draw at the
+    last step cannot shift the stream."""
+    ia, ja, a = driver.laplacian3d(9, 1, 1)
+    ia2, ja2, a2 = driver.laplacian3d(4, 5, 1)
+    # a 20 x 20 matrix with distinct eigenvalues: the 9-chain and the 4x5 grid
+    import scipy.sparse as sps
+    M = sps.block_diag([sps.csr_matrix((a, ja, ia)), sps.csr_matrix((a2, ja2, ia2)) * 0.9],
+                       format="csr")
+    M = (M + sps.diags(np.linspace(0.0, 0.3, M.shape[0]))).tocsr()
+    M.sort_indices()
+    A = orc.Csr(M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data)
+    n = M.shape[0]
+    lam, U = np.linalg.eigh(M.toarray())
+    intv = [lam[0], lam[-1], lam[0] - 0.2, lam[-1] + 0.2]
+    orc.srand(0)
+    rc, xd, yd, neig = orc.landos(A, 1, n, 400, intv)
+    assert rc == 0
+    orc.srand(0)
+    v = orc.rand_vec(n)
+    v = v / np.linalg.norm(v)
+    w = (U.T @ v) ** 2
+    Mq = min(n, 30)
+    sigma = (intv[3] - intv[2]) / (Mq - 1) / math.sqrt(8 * math.log(1.25))
+    width = sigma * math.sqrt(-2 * math.log(1e-8))
+    ref = np.zeros_like(xd)
+    for lj, wj in zip(lam, w):
+        sel = np.abs(xd - lj) < width
+        ref[sel] += wj * np.exp(-((xd[sel] - lj) ** 2) / (2 * sigma * sigma))
+    ref /= math.sqrt(2 * sigma * sigma * math.pi)
+    assert np.max(np.abs(yd - ref)) < 1e-10 * np.max(ref)
