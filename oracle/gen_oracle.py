@@ -2,7 +2,7 @@
 path: the least-squares Chebyshev fit of the B-solves, the generalized
 Chebyshev filter and the generalized KPM moments.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker.  The product
+TEST INFRASTRUCTURE ONLY: imported by tests/ to check results.  The product
 (pevsl_amd) never imports it.
 
 How each function is pinned (tests/test_gen_oracle.py):

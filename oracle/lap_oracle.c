@@ -1,6 +1,6 @@
 /*
  * lap_oracle.c -- CPU oracle driver for the Laplacian spectrum-slicing run.
- * TEST INFRASTRUCTURE ONLY (checker and CPU baseline; see oracle.h).
+ * TEST INFRASTRUCTURE ONLY (result checks and CPU baseline; see oracle.h).
  *
  * Follows the flow of TESTS/Lap/LapPLanN.c:15-323 for one process and one
  * group: srand(0) (SRC/pevsl.c:36 with rank 0), KPM DOS (Mdeg 300, Jackson,

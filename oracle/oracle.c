@@ -1,7 +1,7 @@
 /*
  * oracle.c -- CPU oracle (FP64, sequential) for the pEVSL spectrum-slicing
  * path.  TEST INFRASTRUCTURE ONLY: loaded by tests/, smoke() and the
- * cpu_baseline leg of bench.py as the checker; never by the product library.
+ * cpu_baseline leg of bench.py, to check results; never by the product library.
  *
  * Every routine restates the reference algorithm at the cited file:line.
  * BLAS level-1/2 calls of the reference are written out as plain loops in the

@@ -2,8 +2,8 @@
  * oracle.h -- CPU oracle for the pEVSL spectrum-slicing path.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C, FP64, single-threaded
- * restatement of the reference algorithm (pEVSL) used as the checker for the
- * HIP implementation in pevsl_amd/.  Only tests/, __graft_entry__.smoke() and
+ * restatement of the reference algorithm (pEVSL) that checks the results of
+ * the HIP implementation in pevsl_amd/.  Only tests/, __graft_entry__.smoke() and
  * bench.py's cpu_baseline leg may load it; the product library never links,
  * calls or falls back to it.
  *

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product (pevsl_amd) never
+cpu_baseline leg of bench.py, to check results.  The product (pevsl_amd) never
 imports it.
 """
 import ctypes as C

@@ -2,7 +2,7 @@
 and backward passes, its loss head, the Adam update, the KPM bin masses, the
 equal-share cuts of a binned DOS, the feature map and the full-KPM judge.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker.  The product
+TEST INFRASTRUCTURE ONLY: imported by tests/ to check results.  The product
 (pevsl_amd) never imports it.
 
 The slicer has no counterpart in the reference; the DOS convention it builds
