@@ -587,3 +587,42 @@ def test_rectangular_parcsr_matvec(lib):
             inst.close()
     finally:
         lib.pEVSL_ParcsrFree(C.byref(A))
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_empty_slice_in_a_spectral_gap(lib, batch):
+    """A slice that lies in a gap of the spectrum (6^3 Laplacian: no
+    eigenvalue in (0.5942, 1.1491)) returns 0 eigenvalues, alone and batched
+    with non-empty neighbours, whose counts stay exact."""
+    banner("empty slice in a spectral gap, 6^3, batch=%s" % batch)
+    r = driver.lap_slicing_run(6, 6, 6, 0.3, 1.6, 3, batch=batch,
+                               sli_override=[0.3, 0.75, 0.95, 1.6])
+    counts = [s["nev"] for s in r["slices"]]
+    assert counts[1] == 0
+    for s in r["slices"]:
+        assert s["nev"] == s["nev_exact"]
+    assert counts[0] > 0 and counts[2] > 0
+
+
+@pytest.mark.parametrize("a,b", [(2.5, 5.5), (4.5, 9.5), (2.5, 9.5)])
+def test_matrix_smaller_than_krylov_dimension(lib, a, b):
+    """2^3 Laplacian (n = 8, eigenvalues 3, 5 x3, 7 x3, 9): the Krylov
+    dimension is capped at n and the breakdown restarts reach the repeated
+    eigenvalues.  Counts equal the oracle's ChebLanNr (the reference
+    algorithm).  On [2.5, 9.5], centred on the frame [0, 12], FindPol's
+    degree-2 filter is even about 6, so p(3) = p(9) and p(5) = p(7): the
+    Ritz vectors mix two eigenvalues and fail the residual test, and the
+    reference algorithm, like this build, accepts none.  Off-centre
+    intervals find every eigenvalue to 1e-12."""
+    banner("n=8 Laplacian [%g, %g]" % (a, b))
+    r = driver.lap_slicing_run(2, 2, 2, a, b, 1)
+    s = r["slices"][0]
+    d = orc.run_driver(["-nx", 2, "-ny", 2, "-nz", 2, "-a", a, "-b", b, "-nslices", 1],
+                       timeout=60)
+    assert s["nev"] == d["slices"][0]["nev"]
+    ex = driver.exact_eig_lap3(2, 2, 2, a, b)
+    if (a, b) == (2.5, 9.5):
+        assert s["nev"] == 0 and len(ex) == 8
+    else:
+        assert s["nev"] == len(ex)
+        np.testing.assert_allclose(np.sort(s["lam"]), ex, rtol=1e-12)
