@@ -146,7 +146,9 @@ def main():
             err = float(np.max(np.abs(lam - ex) / np.abs(ex))) if len(lam) == len(ex) and len(ex) else None
             res.append({"slice": s, "found": len(lam), "exact": len(ex), "deg": pols[s].deg,
                         "its": o["info"]["its"], "max_rel_err": err,
-                        "max_res": float(np.max(o["res"])) if len(o["res"]) else None})
+                        "max_res": float(np.max(o["res"])) if len(o["res"]) else None,
+                        "max_rel_res": float(np.max(np.asarray(o["res"]) / np.abs(o["lam"])))
+                        if len(o["res"]) else None})
             log("slice %d: found %d exact %d its %d deg %d err %s" % (
                 s, len(lam), len(ex), o["info"]["its"], pols[s].deg, err))
         return res
@@ -209,6 +211,7 @@ def main():
                  "found": mfound, "exact": sum(r["exact"] for r in mres),
                  "max_rel_err": max((r["max_rel_err"] or 0.0) for r in mres),
                  "max_residual": max((r["max_res"] or 0.0) for r in mres),
+                 "max_rel_residual": max((r["max_rel_res"] or 0.0) for r in mres),
                  "tolerance": "residual < 1e-6 * max(|lmin|, |lmax|) = 1.2e-5 (FP32-filtered basis), "
                               "eigenvalue rel err < 1e-10",
                  "phase_s": {"filter": round(ksm["t_filter"], 3), "cgs": round(ksm["t_cgs"], 3),
@@ -240,6 +243,7 @@ def main():
         "correctness": {"found": found, "exact": sum(r["exact"] for r in results),
                         "max_rel_err": max((r["max_rel_err"] or 0.0) for r in results),
                         "max_residual": max((r["max_res"] or 0.0) for r in results),
+                        "max_rel_residual": max((r["max_rel_res"] or 0.0) for r in results),
                         "per_slice": results},
         "setup_s": round(t_setup, 3), "kpm_slicing_s": round(t_kpm, 3),
         "ecount": ecount,
