@@ -190,6 +190,15 @@ def main():
     # half of a pass) takes longer
     roof = dict(filt) if ks["t_filter"] >= 0.5 * ks["t_cgs"] else dict(cgs)
     roof["traffic_source"] = traffic.get("_source")
+    # what a plain copy kernel reaches on arrays beyond the 256 MB Infinity
+    # Cache (2 x 2 GiB, HIP events): the practical ceiling beside the paper peak
+    csec, cbytes = C.c_double(0.0), C.c_double(0.0)
+    if lib.pEVSL_BenchCopy(C.c_size_t(1 << 28), 10, C.byref(csec), C.byref(cbytes)) == 0 \
+            and csec.value > 0:
+        copy_bw = cbytes.value / csec.value / 1e9
+        for r_ in (roof, filt, cgs):
+            r_["copy_GBs"] = round(copy_bw, 1)
+            r_["frac_of_copy"] = round(r_["achieved"] / copy_bw, 4)
     mixed = None
     if not args.no_mixed:
         instm = driver.Instance(lib, M, prec=capi.PEVSL_PREC_FP32,
