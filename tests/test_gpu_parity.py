@@ -626,3 +626,101 @@ def test_matrix_smaller_than_krylov_dimension(lib, a, b):
     else:
         assert s["nev"] == len(ex)
         np.testing.assert_allclose(np.sort(s["lam"]), ex, rtol=1e-12)
+
+
+def _lap2d_with_empty_rows(m=40):
+    """m x m 5-point Laplacian with the rows and columns of every 11th index
+    and of the contiguous range [600, 1200) removed: about 690 empty rows,
+    rows 768-1023 forming a whole CSR-stream row block (256 rows) with no
+    entry.  Its spectrum is that of the remaining principal submatrix plus
+    zeros."""
+    import scipy.sparse as sp
+    ia, ja, a = driver.laplacian3d(m, m, 1)
+    n = len(ia) - 1
+    L = sp.csr_matrix((a, ja, ia), shape=(n, n))
+    keep = np.ones(n)
+    keep[::11] = 0.0
+    keep[600:1200] = 0.0
+    D = sp.diags(keep)
+    L = (D @ L @ D).tocsr()
+    L.eliminate_zeros()
+    L.sort_indices()
+    return (L.indptr.astype(np.int32), L.indices.astype(np.int32), L.data.astype(np.float64),
+            np.linalg.eigvalsh(L.toarray()))
+
+
+@pytest.mark.parametrize("prec", [capi.PEVSL_PREC_FP64, capi.PEVSL_PREC_FP32])
+def test_cheb_av_empty_rows_against_oracle(lib, prec):
+    """Stream-kernel filter on a matrix with empty rows and empty row blocks:
+    bitwise with the oracle in FP64 (one sequential sum per row), 1e-4 of the
+    filtered vector's size in FP32."""
+    banner("ChebAv with empty rows, prec %d" % prec)
+    ia, ja, a, _ = _lap2d_with_empty_rows()
+    assert ia[1200] == ia[600]
+    M = driver.Matrix(lib, ia, ja, a)
+    iv = [0.5, 1.5, 0.0, 8.0]
+    pol, rc = driver.find_pol(lib, iv)
+    opol = orc.Pol(iv)
+    assert rc == 0 and pol.deg == opol.deg
+    inst = driver.Instance(lib, M, prec=prec, lanes=capi.PEVSL_SPMV_STREAM)
+    try:
+        v = np.random.default_rng(3).uniform(-1, 1, M.n)
+        y = np.zeros(M.n)
+        rc = lib.pEVSL_ChebAv(inst.p, C.byref(pol), C.byref(parvec(lib, v)),
+                              C.byref(parvec(lib, y)), None)
+        assert rc == 0
+    finally:
+        inst.close()
+        lib.pEVSL_FreePol(C.byref(pol))
+        M.free()
+    ref = orc.cheb_av(orc.Csr(ia, ja, a), opol, v)
+    if prec == capi.PEVSL_PREC_FP64:
+        assert np.array_equal(y, ref)
+    else:
+        assert np.max(np.abs(y - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("prec,basis,rowk,batch", [
+    (capi.PEVSL_PREC_FP64, capi.PEVSL_PREC_FP64, -1, False),
+    (capi.PEVSL_PREC_FP64, capi.PEVSL_PREC_FP64, 1, True),
+    (capi.PEVSL_PREC_FP64, capi.PEVSL_PREC_FP64, 3, True),
+    (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, -1, False),
+    (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, 2, True),
+    (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, 4, True)])
+def test_cheblan_empty_rows_counts_match_dense(lib, prec, basis, rowk, batch):
+    """ChebLanNr on the matrix with empty rows, one slice at a time (stream
+    kernel) or as a batch of two (row kernels 1 / 2, nnz-balanced kernels
+    3 / 4): every slice's count equals the dense count, eigenvalues to 1e-10
+    relative; residuals below 1e-8 (FP64) or 1e-6 * 8 (FP32 basis)."""
+    banner("ChebLanNr with empty rows, prec %d basis %d rowk %d batch %s"
+           % (prec, basis, rowk, batch))
+    ia, ja, a, ev = _lap2d_with_empty_rows()
+    M = driver.Matrix(lib, ia, ja, a)
+    inst = driver.Instance(lib, M, prec=prec, basis_prec=basis, block_rowkernel=rowk,
+                           lanes=capi.PEVSL_SPMV_STREAM)
+    edges = [0.5, 1.0, 1.5]
+    try:
+        intvs = [[edges[i], edges[i + 1], 0.0, 8.0] for i in range(2)]
+        pols = [driver.find_pol(lib, iv)[0] for iv in intvs]
+        counts = [int(np.sum((ev >= iv[0]) & (ev <= iv[1]))) for iv in intvs]
+        maxits = [min(max(5 * (c + 2), 300), M.n) for c in counts]
+        vinit = driver.rand_vinit(lib, M.n)
+        groups = [[0, 1]] if batch else [[0], [1]]
+        outs = []
+        for g in groups:
+            rc, o = driver.cheblan_batch(inst, [intvs[i] for i in g], [maxits[i] for i in g],
+                                         1e-8, vinit, [pols[i] for i in g])
+            assert rc == 0
+            outs += o
+        for p_ in pols:
+            lib.pEVSL_FreePol(C.byref(p_))
+    finally:
+        inst.close()
+        M.free()
+    tol_res = 1e-8 if basis == capi.PEVSL_PREC_FP64 else 1e-6 * 8.0
+    for iv, c, o in zip(intvs, counts, outs):
+        lam = np.sort(o["lam"])
+        ex = ev[(ev >= iv[0]) & (ev <= iv[1])]
+        assert len(lam) == c > 0, (iv, len(lam), c)
+        assert np.max(np.abs(lam - ex) / np.abs(ex)) < 1e-10
+        assert np.all(np.asarray(o["res"]) < tol_res)
