@@ -3,6 +3,7 @@ runs it, on one GPU, with a JSON summary on stdout (the last line).
 
     python -m pevsl_amd.lap_slicing --mode fp64              # one slice at a time
     python -m pevsl_amd.lap_slicing --mode mixed-mlp --wave 20
+    python -m pevsl_amd.lap_slicing --mode mixed-mlp --slices 0-7   # a subset
 
 Modes:
   fp64       FP64 filter and basis, KPM slicing, slices solved one at a time
@@ -40,14 +41,25 @@ def main(argv=None):
     ap.add_argument("--wave", type=int, default=0,
                     help="slices per batched call (0: all); the FP32 bases of one wave must fit in HBM")
     ap.add_argument("--weights", default=WEIGHTS, help="learned-slicer weights (mixed-mlp)")
+    ap.add_argument("--slices", default="",
+                    help="solve only these slices of the slicing, e.g. 0-7 or 0,5,9 (default: all)")
+    ap.add_argument("--verbose", action="store_true", help="Lanczos progress every 20 steps")
     ap.add_argument("--quiet", action="store_true")
     a = ap.parse_args(argv)
+    only = None
+    if a.slices:
+        only = []
+        for part in a.slices.split(","):
+            lo, _, hi = part.partition("-")
+            only += list(range(int(lo), int(hi or lo) + 1))
+        if not only or min(only) < 0 or max(only) >= a.nslices:
+            ap.error("--slices must name slices in [0, %d)" % a.nslices)
 
     def log(m):
         if not a.quiet:
             print(m, file=sys.stderr, flush=True)
 
-    kw = dict(nvec=60, Mdeg=300, log=log)
+    kw = dict(nvec=60, Mdeg=300, log=log, slices_only=only, verbose=a.verbose)
     if a.mode == "fp64":
         kw.update(prec=capi.PEVSL_PREC_FP64, batch=False)
     else:
@@ -67,6 +79,7 @@ def main(argv=None):
         "workload": "Laplacian %d^3, [%g, %g] in %d slices" % (g, a.interval[0], a.interval[1],
                                                              a.nslices),
         "mode": a.mode, "wave": a.wave or a.nslices if a.mode != "fp64" else 1,
+        "slices_solved": [d["slice"] for d in sl],
         "wall_s": round(wall, 2), "lanczos_s": round(r["t_lanczos"], 2),
         "dos_slicing_s": round(r["t_dos"], 3), "findpol_s": round(r["t_findpol"], 3),
         "found": int(sum(d["nev"] for d in sl)), "exact": int(sum(d["nev_exact"] for d in sl)),
