@@ -43,10 +43,13 @@ def log(*a):
 
 
 def cpu_baseline(grid, a, b, nev_slice, its, nev_found, t_budget=25.0):
-    """Times the FP64 CPU oracle (1 core) on a bounded sample of the same
-    slice: a few filtered Lanczos steps and CGS passes at k = 512, then
-    projects the full slice from the GPU-measured step count `its`:
-        T = its * t_step + t_cgs(512)/512 * its^2/2 ."""
+    """Times the FP64 CPU oracle on a bounded sample of the same slice: a few
+    filtered Lanczos steps and CGS passes at k = 512, then projects the full
+    slice from the GPU-measured step count `its`:
+        T = its * t_step + t_cgs(512)/512 * its^2/2 .
+    The oracle's long loops (CSR product, vector updates, CGS) run on
+    OMP_NUM_THREADS host threads with results independent of the count;
+    `cores` reports the count it ran on."""
     from oracle import oracle as orc
     orc.build() if not os.path.exists(orc.DRIVER) else None
     steps = 2
@@ -61,11 +64,12 @@ def cpu_baseline(grid, a, b, nev_slice, its, nev_found, t_budget=25.0):
     t_step = s["t_lan"] / (steps + 1)
     t_cgs = d["t_cgs_pass"]
     T = its * t_step + (t_cgs / 512.0) * its * its / 2.0
-    return {"value": nev_found / T, "unit": "eigenpairs/s", "cores": 1, "kind": "port",
-            "sample": ("oracle (FP64 C, 1 thread) on the same slice [%.6f, %.6f]: %d filtered "
-                       "Lanczos steps (deg %d) + 3 CGS passes at k=512, %.1f s wall; projected "
-                       "to the GPU run's %d steps: %.0f s per slice" % (
-                           a, b, steps, s["deg"], wall, its, T)),
+    threads = int(d.get("threads", 1))
+    return {"value": nev_found / T, "unit": "eigenpairs/s", "cores": threads, "kind": "port",
+            "sample": ("oracle (FP64 C, %d OpenMP threads) on the same slice [%.6f, %.6f]: %d "
+                       "filtered Lanczos steps (deg %d) + 3 CGS passes at k=512, %.1f s wall; "
+                       "projected to the GPU run's %d steps: %.0f s per slice" % (
+                           threads, a, b, steps, s["deg"], wall, its, T)),
             "t_step_s": t_step, "t_cgs512_s": t_cgs, "projected_slice_s": T}
 
 

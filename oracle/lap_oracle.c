@@ -16,6 +16,8 @@
  *                      1), nev taken as -nev (timing samples)
  *          -reorthk K: also time three CGS-DGKS passes against K basis
  *                      columns ("t_cgs_pass" in the output)
+ * The output's "threads" is the OpenMP thread count the long loops of the
+ * oracle ran on (OMP_NUM_THREADS); results do not depend on it.
  */
 #include <float.h>
 #include <math.h>
@@ -23,6 +25,8 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+
+#include <omp.h>
 
 #include "oracle.h"
 
@@ -175,7 +179,7 @@ int main(int argc, char **argv) {
     free(v);
     free(w);
   }
-  printf(", \"t_total\": %.6f}\n", wtime() - t0);
+  printf(", \"threads\": %d, \"t_total\": %.6f}\n", omp_get_max_threads(), wtime() - t0);
   free(mu);
   free(sli);
   free(vinit);

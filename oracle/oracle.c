@@ -1,5 +1,5 @@
 /*
- * oracle.c -- CPU oracle (FP64, sequential) for the pEVSL spectrum-slicing
+ * oracle.c -- CPU oracle (FP64) for the pEVSL spectrum-slicing
  * path.  TEST INFRASTRUCTURE ONLY: loaded by tests/, smoke() and the
  * cpu_baseline leg of bench.py, to check results; never by the product library.
  *
@@ -11,6 +11,14 @@
  * SRC/miscla.c:29-66) is an implicit-shift QL iteration (Bowdler, Martin,
  * Reinsch & Wilkinson, Numer. Math. 11 (1968) 293-306), pinned against
  * scipy.linalg.eigh_tridiagonal in tests/.
+ *
+ * Built with OpenMP, the O(n) and O(n k) loops (CSR product, axpy, scal, the
+ * CGS dot products and update) split over threads for n >= ORC_PAR_MIN in a
+ * way that leaves every element's sequence of operations as in the
+ * sequential loops: rows or elements are divided among threads, each dot
+ * product stays one sequential sum, and the CGS update keeps its column
+ * order per element.  Results are therefore bitwise the same for any
+ * thread count.  Other dot products (norms, Lanczos alpha) stay sequential.
  */
 #include "oracle.h"
 
@@ -26,6 +34,7 @@
 
 #define ORC_ORTH_TOL 1e-14 /* INC/pevsl_def.h:18 orthTol */
 #define ORC_NGS_MAX 2      /* INC/pevsl_def.h:112 NGS_MAX */
+#define ORC_PAR_MIN 65536  /* vectors at least this long use threads */
 
 static void *xmalloc(size_t n) {
   void *p = malloc(n ? n : 1);
@@ -42,9 +51,11 @@ static double dot(int n, const double *x, const double *y) {
   return s;
 }
 static void axpy(int n, double a, const double *x, double *y) {
+#pragma omp parallel for schedule(static) if (n >= ORC_PAR_MIN)
   for (int i = 0; i < n; i++) y[i] += a * x[i];
 }
 static void scal(int n, double a, double *x) {
+#pragma omp parallel for schedule(static) if (n >= ORC_PAR_MIN)
   for (int i = 0; i < n; i++) x[i] *= a;
 }
 
@@ -94,6 +105,7 @@ void orc_csr_free(orc_csr *A) {
 
 /* y = A x, one sequential sum per row: SRC/spmat.c:184-206 (dcsrmv 'N'). */
 void orc_csrmv(const orc_csr *A, const double *x, double *y) {
+#pragma omp parallel for schedule(static) if (A->nrows >= ORC_PAR_MIN)
   for (int i = 0; i < A->nrows; i++) {
     double r = 0.0;
     for (int j = A->ia[i]; j < A->ia[i + 1]; j++) r += A->a[j] * x[A->ja[j]];
@@ -262,8 +274,17 @@ void orc_cgs_dgks(int n, int k, int i_max, const double *V, int ld, double *v,
   const double eta = 1.0 / sqrt(2.0);
   double old_nrm = sqrt(dot(n, v, v)), new_nrm = old_nrm;
   for (int it = 0; it < i_max; it++) {
+#pragma omp parallel for schedule(static) if (n >= ORC_PAR_MIN && k > 1)
     for (int j = 0; j < k; j++) work[j] = dot(n, V + (size_t)j * ld, v);
-    for (int j = 0; j < k; j++) axpy(n, -work[j], V + (size_t)j * ld, v);
+    /* v -= V work, column by column for every element (netlib DGEMV 'N') */
+#pragma omp parallel for schedule(static) if (n >= ORC_PAR_MIN)
+    for (int i0 = 0; i0 < n; i0 += 1024) {
+      const int i1 = i0 + 1024 < n ? i0 + 1024 : n;
+      for (int j = 0; j < k; j++) {
+        const double a = -work[j], *x = V + (size_t)j * ld;
+        for (int i = i0; i < i1; i++) v[i] += a * x[i];
+      }
+    }
     new_nrm = sqrt(dot(n, v, v));
     if (new_nrm > eta * old_nrm) break;
     old_nrm = new_nrm;

@@ -1,34 +1,42 @@
 /*
  * oracle.h -- CPU oracle for the pEVSL spectrum-slicing path.
  *
- * TEST INFRASTRUCTURE ONLY.  This is a plain-C, FP64, single-threaded
- * restatement of the reference algorithm (pEVSL) that checks the results of
- * the HIP implementation in pevsl_amd/.  Only tests/, __graft_entry__.smoke() and
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C, FP64 restatement of the
+ * reference algorithm (pEVSL) that checks the results of the HIP
+ * implementation in pevsl_amd/.  Only tests/, __graft_entry__.smoke() and
  * bench.py's cpu_baseline leg may load it; the product library never links,
- * calls or falls back to it.
+ * calls or falls back to it.  Its long loops may run on OpenMP threads;
+ * results are bitwise independent of the thread count (oracle.c).
  *
  * Each function cites the reference file:line whose behaviour it restates.
- * Pinning (how each function is checked independently of this restatement):
- *   orc_lap3d / orc_exact_eig_lap3 : closed form of the 7-point Laplacian
- *                                    (TESTS/COMMON/lapl.c:139-169) and
- *                                    brute-force dense eigvalsh on small grids.
- *   orc_csrmv                      : integer stencil arithmetic, x_i = sin(i)
- *                                    recipe of TESTS0/Matvec/test1.c:70-94.
+ * The reference itself cannot be built here (it needs MPI and BLAS/LAPACK),
+ * so each function is pinned independently of this restatement by tests/:
+ *   orc_lap3d / orc_exact_eig_lap3 : dense eigvalsh of the generated matrix
+ *                                    equals the closed form
+ *                                    (TESTS/COMMON/lapl.c:139-169).
+ *   orc_csrmv                      : x_i = sin(i) recipe of
+ *                                    TESTS0/Matvec/test1.c:70-94 vs scipy.
  *   orc_tridiag_eig                : scipy.linalg.eigh_tridiagonal (LAPACK).
- *   orc_dampcf / orc_chebxPltd     : closed forms (Jackson/sigma factors,
- *                                    T_k(cos t) = cos(k t)).
- *   orc_kpmdos / orc_spslicer_kpm  : golden vectors from the reference built
- *                                    from its own sources (oracle/_ref), exact
- *                                    moments on a diagonal matrix, count 118.
- *   orc_find_pol                   : golden vectors (oracle/_ref) and the
- *                                    invariants p(gam)=1, p(a)=p(b)=bar.
+ *   orc_dampcf / orc_chebxPltd     : the textbook Jackson and sigma factors;
+ *                                    numpy's Chebyshev evaluation.
+ *   orc_kpmdos                     : exact moments of a diagonal matrix by
+ *                                    the spectral sum; the C1 count 118
+ *                                    within the stochastic error.
+ *   orc_intchx / orc_spslicer_kpm  : numerical quadrature of the DOS; equal
+ *                                    shares; monotone slices.
+ *   orc_find_pol                   : the README degrees (README.md:111-115)
+ *                                    and the invariants p(gam)=1,
+ *                                    p(a)=p(b)=bar.
  *   orc_cheb_av                    : p(A)v on a diagonal matrix == scalar
  *                                    evaluation of p at each diagonal entry.
- *   orc_cheblan_nr                 : per-slice counts and eigenvalues vs the
- *                                    closed form and the README run
- *                                    (README.md:111-115), golden vectors.
- *   orc_landos / orc_spslicer_lan  : golden vectors (oracle/_ref); Simpson
- *                                    rule exact on cubics.
+ *   orc_cgs_dgks                   : orthogonality; the exact projection.
+ *   orc_cheblan_nr                 : the README counts per slice, the closed
+ *                                    form to 1e-12, eigenvector residuals.
+ *   orc_landos / orc_spslicer_lan  : full-length Lanczos equals the spectral
+ *                                    measure of the start vector; Simpson
+ *                                    exact on cubics.
+ *   orc_sym_eig / orc_lan_trbounds : numpy.linalg.eigvalsh; closed-form
+ *                                    extremes of Laplacians enclosed.
  */
 #ifndef PEVSL_ORACLE_H
 #define PEVSL_ORACLE_H

@@ -286,11 +286,15 @@ def exact_eig_lap3(nx, ny, nz, a, b):
     return out[:cnt]
 
 
-def run_driver(args, timeout=None):
-    """Run the oracle driver binary (LapPLanN.c flow); returns parsed JSON."""
+def run_driver(args, timeout=None, threads=None):
+    """Run the oracle driver binary (LapPLanN.c flow); returns parsed JSON.
+    `threads` sets OMP_NUM_THREADS for the run (default: inherited)."""
     import json
     if not os.path.exists(DRIVER):
         build()
+    env = None
+    if threads is not None:
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     r = subprocess.run([DRIVER] + [str(a) for a in args], capture_output=True,
-                       text=True, check=True, timeout=timeout)
+                       text=True, check=True, timeout=timeout, env=env)
     return json.loads(r.stdout)

@@ -370,3 +370,25 @@ def test_landos_full_lanczos_is_the_spectral_measure():
         ref[sel] += wj * np.exp(-((xd[sel] - lj) ** 2) / (2 * sigma * sigma))
     ref /= math.sqrt(2 * sigma * sigma * math.pi)
     assert np.max(np.abs(yd - ref)) < 1e-10 * np.max(ref)
+
+
+def test_driver_output_independent_of_thread_count():
+    """The oracle's threaded loops (CSR product, vector updates, CGS) keep
+    every element's sequence of operations, so a run on 41^3 (n = 68,921,
+    above the threading threshold) prints bitwise the same moments, slices,
+    eigenvalues and residuals on 1 and 4 threads."""
+    args = ["-nx", 41, "-ny", 41, "-nz", 41, "-a", 0.6, "-b", 0.7, "-nslices", 2, "-Mdeg", 30,
+            "-nvec", 4, "-slice", 0, "-maxsteps", 80, "-reorthk", 64]
+    r1 = orc.run_driver(args, timeout=300, threads=1)
+    r4 = orc.run_driver(args, timeout=300, threads=4)
+    assert r1["threads"] == 1 and r4["threads"] == 4
+
+    def strip(d):
+        if isinstance(d, dict):
+            return {k: strip(v) for k, v in d.items() if not k.startswith("t_") and k != "threads"}
+        if isinstance(d, list):
+            return [strip(x) for x in d]
+        return d
+
+    assert strip(r1) == strip(r4)
+    assert r1["slices"][0]["nev"] > 0
