@@ -163,7 +163,7 @@ int pEVSL_Start(MPI_Comm comm, pevsl_Data **data) {
   p->n = PEVSL_NOT_DEFINED;
   p->nfirst = PEVSL_NOT_DEFINED;
   p->sigma_mult = 1.0;
-  // one process: rank 0 seeds the rand() stream (SRC/pevsl.c:331-336)
+  // one process: rank 0 seeds the rand() stream (SRC/pevsl.c:31-36)
   g_rand.seed(0);
   PEVSL_MALLOC(p->stats, 1, pevsl_Stat);
   Ctx *c = new Ctx();
