@@ -70,7 +70,7 @@ int pEVSL_LanTrbounds(pevsl_Data *pevsl, int lanm, int maxit, double tol,
  * pEVSL_SetLTSol it runs on the device with no host copies) and its
  * release.  The fit is host work; solves need B set up with
  * pEVSL_ParcsrSetup (its HBM copy).
- *                                       SRC/lspolapprox.c:63-146, 190-322 */
+ *                                       SRC/lspolapprox.c:63-146, 221-321 */
 void pEVSL_SetupLSPolSqrt(int max_deg, double tol, double lmin, double lmax,
                           pevsl_Parcsr *B, void **vdata);
 void pEVSL_LSPolSol(double *db, double *dx, void *data);

@@ -93,7 +93,7 @@ int pEVSL_SetRandMode(pevsl_Data *pevsl, int mode, unsigned long long seed);
 int pEVSL_SetKeepVectors(pevsl_Data *pevsl, int keep);
 
 /* Least-squares Chebyshev approximations for generalized problems
- * (SRC/lspolapprox.c:63-190): of B^{-1} (SetupLSPolRec, a B-solve), and of
+ * (SRC/lspolapprox.c:63-210): of B^{-1} (SetupLSPolRec, a B-solve), and of
  * any positive function ffun of B (SetupLSPol).  pEVSL_LSPolInfo returns the
  * chosen degree, the deg + 1 coefficients (mu may be NULL) and the interval
  * map x -> (x - cc)/dd.  Free with pEVSL_LSPolFree. */

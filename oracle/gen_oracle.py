@@ -59,7 +59,7 @@ def cheb_series(mu, x):
 
 
 def lspol_apply(B, mu, cc, dd, b):
-    """sum_k mu_k T_k((B - cc)/dd) b (pnav, SRC/lspolapprox.c:222-268)."""
+    """sum_k mu_k T_k((B - cc)/dd) b (pnav, SRC/lspolapprox.c:245-288)."""
     vk, vkm1 = b.copy(), np.zeros_like(b)
     y = mu[0] * vk
     for k in range(1, len(mu)):

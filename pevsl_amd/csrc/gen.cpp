@@ -42,7 +42,7 @@ struct LSPolData {
 namespace {
 
 // y = sum_{k <= deg} mu_k T_k((B - cc)/dd) x on the device (the recurrence of
-// pnav, SRC/lspolapprox.c:222-268, fused per degree into one pass).
+// pnav, SRC/lspolapprox.c:245-288, fused per degree into one pass).
 void lspol_apply(const LSPolData &L, const double *dx, double *dy, double *wk, hipStream_t s) {
   const DevCsr *Bd = dev_matrix_of(L.Bp);
   if (!Bd) fatal("LSPol: B has no device copy (pEVSL_ParcsrSetup)", __FILE__, __LINE__);

@@ -319,8 +319,8 @@ int pEVSL_LanTrbounds(pevsl_Data *pevsl, int lanm, int maxit, double tol, pevsl_
     PV_HIP(hipStreamSynchronize(st));
     return v;
   };
-  // vnew = filtered random vector orthogonalized against V(:,0:k-1), unit
-  // norm, beta = 0 (lucky breakdown, SRC/lantrbnd.c:224-246, 179-196)
+  // vnew = random vector orthogonalized against V(:,0:k-1), unit norm,
+  // beta = 0 (lucky breakdown, SRC/lantrbnd.c:192-211, 278-301)
   auto restart_random = [&](int k) {
     rand_vectors(c, n, 1, col(k), ld);
     set_dev(L.alpha, 0.0);
@@ -348,7 +348,7 @@ int pEVSL_LanTrbounds(pevsl_Data *pevsl, int lanm, int maxit, double tol, pevsl_
       break;
     }
     if (trlen > 0) {
-      // thick-restart step (SRC/lantrbnd.c:142-208): arrowhead T
+      // thick-restart step (SRC/lantrbnd.c:142-223): arrowhead T
       const int k1 = k - 1;
       apply_A(c, col(k1), col(k1 + 1));
       for (int i = 0; i < trlen; i++) {
@@ -376,7 +376,7 @@ int pEVSL_LanTrbounds(pevsl_Data *pevsl, int lanm, int maxit, double tol, pevsl_
       Tat(k, k1) = beta;
       Tat(k1, k) = beta;
     }
-    // Lanczos steps with full reorthogonalization (SRC/lantrbnd.c:214-288)
+    // Lanczos steps with full reorthogonalization (SRC/lantrbnd.c:236-314)
     while (k < lanm && it < maxit) {
       k++;
       apply_A(c, col(k - 1), col(k));
@@ -406,7 +406,7 @@ int pEVSL_LanTrbounds(pevsl_Data *pevsl, int lanm, int maxit, double tol, pevsl_
       Tat(k, k - 1) = beta;
       Tat(k - 1, k) = beta;
     }
-    // eigenpairs of T(0:k, 0:k) (SRC/lantrbnd.c:293-313)
+    // eigenpairs of T(0:k, 0:k) (SRC/lantrbnd.c:315-324)
     Tk.assign((size_t)k * k, 0.0);
     for (int cc = 0; cc < k; cc++)
       for (int r = 0; r < k; r++) Tk[(size_t)cc * k + r] = Tat(r, cc);
