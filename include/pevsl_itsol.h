@@ -18,11 +18,17 @@
 extern "C" {
 #endif
 
+/* EXTERNAL/ITERSOL/pevsl_itsol.h:15 */
 int pEVSL_ChebIterSetup(double lmin, double lmax, int deg, pevsl_Parcsr *A, void **data);
+/* EXTERNAL/ITERSOL/pevsl_itsol.h:18 */
 void pEVSL_ChebIterSolv1(double *db, double *dx, void *data);
+/* EXTERNAL/ITERSOL/pevsl_itsol.h:20 */
 void pEVSL_ChebIterSolv2(double *db, double *dx, void *data);
+/* EXTERNAL/ITERSOL/pevsl_itsol.h:22 */
 void pEVSL_ChebIterFree(void *vdata);
+/* EXTERNAL/ITERSOL/pevsl_itsol.h:24 */
 double *pEVSL_ChebIterGetRes(void *data);
+/* EXTERNAL/ITERSOL/pevsl_itsol.h:26 */
 void pEVSL_ChebIterStatsPrint(void *data, FILE *fstats);
 
 #ifdef __cplusplus
