@@ -105,7 +105,8 @@ typedef struct _pevsl_polparams {
   int deg;           /* 0 on input: degree chosen by FindPol */
 } pevsl_Polparams;
 
-/* callback prototypes (INC/pevsl_struct.h:190-208); host arrays */
+/* callback prototypes (INC/pevsl_struct.h:190-208, the host-array forms) and
+ * their holders (INC/pevsl_struct.h:214-239) */
 typedef void (*SVFuncC)(double *br, double *bz, double *xr, double *xz,
                         void *data);
 typedef void (*SVFunc)(double *b, double *x, void *data);
