@@ -49,11 +49,14 @@ def main(argv=None):
     only = None
     if a.slices:
         only = []
-        for part in a.slices.split(","):
-            lo, _, hi = part.partition("-")
-            only += list(range(int(lo), int(hi or lo) + 1))
+        try:
+            for part in a.slices.split(","):
+                lo, _, hi = part.partition("-")
+                only += list(range(int(lo), int(hi or lo) + 1))
+        except ValueError:
+            only = []
         if not only or min(only) < 0 or max(only) >= a.nslices:
-            ap.error("--slices must name slices in [0, %d)" % a.nslices)
+            ap.error("--slices must name slices in [0, %d), e.g. 0-7 or 0,5,9" % a.nslices)
 
     def log(m):
         if not a.quiet:
