@@ -100,7 +100,10 @@ int pEVSL_SetSigmaMult(pevsl_Data *pevsl_data, double mult);
 
 /* ---- parcsr.c (INC/pevsl.h:54-61) ----------------------------------------
  * One rank: row_starts/col_starts must be NULL or {0, n}; Setup keeps the
- * whole matrix in diag (offd has zero columns).        SRC/parcsr.c:24-304 */
+ * whole matrix in diag (offd has zero columns).  Setup returns -1 for a CSR
+ * of another row count and -2 for a malformed one (ia[0] != 0, ia
+ * decreasing, or a column outside [0, ncol)), before allocating anything.
+ *                                                      SRC/parcsr.c:24-304 */
 int pEVSL_ParcsrCreate(int nrow, int ncol, int *row_starts, int *col_starts,
                        pevsl_Parcsr *A, MPI_Comm comm);
 int pEVSL_ParcsrSetup(pevsl_Csr *Ai, pevsl_Parcsr *A);

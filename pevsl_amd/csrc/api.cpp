@@ -366,18 +366,25 @@ int pEVSL_ParcsrCreate(int nrow, int ncol, int *row_starts, int *col_starts,
 }
 
 int pEVSL_ParcsrSetup(pevsl_Csr *Ai, pevsl_Parcsr *A) {
-  check_device();
   const int n = A->nrow_local;
-  if (Ai->nrows != n) return -1;
+  if (!Ai || Ai->nrows != n) return -1;
+  // The kernels index x and their LDS staging with ia/ja: a malformed CSR
+  // (ia not starting at 0 or decreasing, a column outside [0, ncol)) is
+  // rejected here, before anything is allocated or reaches the device.
+  if (n < 0 || !Ai->ia || Ai->ia[0] != 0) return -2;
+  for (int i = 0; i < n; i++)
+    if (Ai->ia[i + 1] < Ai->ia[i]) return -2;
   const int nnz = Ai->ia[n];
+  if (nnz > 0 && (!Ai->ja || !Ai->a)) return -2;
+  for (int j = 0; j < nnz; j++)
+    if (Ai->ja[j] < 0 || Ai->ja[j] >= A->ncol_local) return -2;
+  check_device();
   // diag = the whole local matrix (all columns are local with one rank)
   PEVSL_MALLOC(A->diag, 1, pevsl_Csr);
   pEVSL_CsrResize(n, A->ncol_local, nnz, A->diag);
   memcpy(A->diag->ia, Ai->ia, sizeof(int) * (size_t)(n + 1));
   memcpy(A->diag->ja, Ai->ja, sizeof(int) * (size_t)nnz);
   memcpy(A->diag->a, Ai->a, sizeof(double) * (size_t)nnz);
-  for (int j = 0; j < nnz; j++)
-    if (A->diag->ja[j] < 0 || A->diag->ja[j] >= A->ncol_local) return -2;
   PEVSL_MALLOC(A->offd, 1, pevsl_Csr);
   pEVSL_CsrResize(n, 0, 0, A->offd);
   for (int i = 0; i <= n; i++) A->offd->ia[i] = 0;

@@ -223,3 +223,28 @@ def test_rand_stream_equals_libc_rand(lib, seed):
     libc.rand()
     lib.pEVSL_ParvecRand(C.byref(pv))
     assert np.array_equal(v, want)
+
+
+def test_malformed_csr_is_rejected_before_the_device(lib):
+    """pEVSL_ParcsrSetup checks ia/ja on the host before any allocation or
+    device call, so malformed matrices return -2 (another row count -1)
+    without a GPU."""
+    import numpy as np
+
+    def setup(n, ia, ja, a, ncol=None):
+        ia = np.ascontiguousarray(ia, dtype=np.int32)
+        ja = np.ascontiguousarray(ja, dtype=np.int32)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        A = capi.Parcsr()
+        assert lib.pEVSL_ParcsrCreate(n, ncol or n, None, None, C.byref(A),
+                                      capi.MPI_COMM_WORLD) == 0
+        csr = capi.Csr(len(ia) - 1, ncol or n, ia.ctypes.data_as(capi.c_int_p),
+                       ja.ctypes.data_as(capi.c_int_p), a.ctypes.data_as(capi.c_double_p))
+        return lib.pEVSL_ParcsrSetup(C.byref(csr), C.byref(A))
+
+    one = [1.0, 1.0, 1.0]
+    assert setup(3, [0, 1, 2, 3], [0, 1, 3], one) == -2     # column 3 of 3
+    assert setup(3, [0, 1, 2, 3], [0, -1, 2], one) == -2    # negative column
+    assert setup(3, [1, 1, 2, 3], [0, 1, 2], one) == -2     # ia[0] != 0
+    assert setup(3, [0, 2, 1, 3], [0, 1, 2], one) == -2     # ia decreasing
+    assert setup(3, [0, 1, 2], [0, 1], one[:2]) == -1       # 2 rows for a 3-row matrix
