@@ -248,3 +248,36 @@ def test_malformed_csr_is_rejected_before_the_device(lib):
     assert setup(3, [1, 1, 2, 3], [0, 1, 2], one) == -2     # ia[0] != 0
     assert setup(3, [0, 2, 1, 3], [0, 1, 2], one) == -2     # ia decreasing
     assert setup(3, [0, 1, 2], [0, 1], one[:2]) == -1       # 2 rows for a 3-row matrix
+
+
+@pytest.mark.parametrize("damping,thresh", [(0, (0.8, 0.2)), (1, (0.8, 0.2)), (2, (0.8, 0.2)),
+                                            (2, (0.6, 0.15))])
+def test_findpol_random_intervals_match_oracle(lib, damping, thresh):
+    """FindPol (host code of the library) against the oracle on 50 seeded
+    random intervals per setting: frames [lmin, lmax] of random centre and
+    width, intervals of every relative width from 1e-4 to the whole frame,
+    including ones that touch an end (left/right filter types), for each
+    damping type and both threshold pairs the reference drivers use
+    (0.8/0.2: Lap, 0.6/0.15: Gen_Lap and MM).  Degree, type and bar agree
+    exactly or to rounding; the coefficients to 1e-13."""
+    rng = np.random.default_rng(100 + damping + int(10 * thresh[0]))
+    for _ in range(50):
+        lmin = rng.uniform(-50, 50)
+        lmax = lmin + 10 ** rng.uniform(-1, 3)
+        w = (lmax - lmin) * 10 ** rng.uniform(-4, 0)
+        a = rng.uniform(lmin, lmax - w)
+        if rng.uniform() < 0.15:
+            a = lmin  # left end
+        elif rng.uniform() < 0.15:
+            a = lmax - w  # right end
+        intv = [a, a + w, lmin, lmax]
+        kw = dict(damping=damping, thresh_int=thresh[0], thresh_ext=thresh[1], max_deg=3000)
+        pol, rc = pol_from_lib(lib, intv, **kw)
+        opol = orc.Pol(intv, **kw)
+        assert rc == opol.rc, intv
+        if rc == 0:
+            assert pol.deg == opol.deg and pol.type == opol.type, intv
+            assert abs(pol.bar - opol.bar) <= 1e-13 * max(1, abs(opol.bar)), intv
+            assert abs(pol.gam - opol.gam) <= 1e-13, intv
+            np.testing.assert_allclose(driver.pol_mu(pol), opol.mu, rtol=1e-12, atol=1e-15)
+        lib.pEVSL_FreePol(C.byref(pol))
