@@ -102,7 +102,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (multi-GPU sharding is "
+        print("bench.py: this library runs on one GPU only (multi-GPU sharding is "
               "designed, not built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 

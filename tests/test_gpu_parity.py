@@ -612,7 +612,7 @@ def test_matrix_smaller_than_krylov_dimension(lib, a, b):
     algorithm).  On [2.5, 9.5], centred on the frame [0, 12], FindPol's
     degree-2 filter is even about 6, so p(3) = p(9) and p(5) = p(7): the
     Ritz vectors mix two eigenvalues and fail the residual test, and the
-    reference algorithm, like this build, accepts none.  Off-centre
+    reference algorithm, like this library, accepts none.  Off-centre
     intervals find every eigenvalue to 1e-12."""
     banner("n=8 Laplacian [%g, %g]" % (a, b))
     r = driver.lap_slicing_run(2, 2, 2, a, b, 1)
