@@ -1,21 +1,37 @@
 // kernels.hip -- hand-written HIP kernels for gfx950 (CDNA4, wave64) behind
 // the pEVSL C API.
 //
-// Hot path (HBM-bound, ~1 flop/byte):
-//   * cheb_step_vec / cheb_step_blk : one degree of the Chebyshev filter
-//     (SRC/chebpol.c:538-568) fused into the sparse product: the SpMV/SpMM,
-//     the three-term recurrence and the coefficient accumulation are one pass
-//     over the matrix and the vectors.  _vec handles one column with L lanes
-//     per row; _blk handles B interleaved columns (row-major n x B) with one
-//     lane per (row, column) so that the column loads of X are contiguous.
-//   * kpm_step_blk : one degree of the KPM recurrence (SRC/kpmdos.c:129-156)
-//     for all random vectors at once, fused with the per-column dot product
+// Hot path (HBM-bound, ~1 flop/byte).  Every filter kernel fuses one degree
+// of the Chebyshev filter (SRC/chebpol.c:538-568) into the sparse product:
+// the SpMV/SpMM, the recurrence and the coefficient accumulation are one pass
+// over the matrix and the vectors.
+//   * k_cheb_stream : one column, CSR-stream form.  A row block (<= 256 rows,
+//     <= 2048 nonzeros) multiplies its nonzeros with the gathered x values in
+//     coalesced order into LDS, then each thread sums one row sequentially
+//     (the reference's order); a longer row is one block, reduced by all of
+//     its threads.  The default single-slice filter.
+//   * k_cheb_vec : one column with L lanes per row (tuning alternative).
+//   * k_cheb_blkrow / k_clen_blkrow : B slices as interleaved columns
+//     (row-major n x B), a/ja staged in LDS, one thread per row gathering
+//     whole B-wide rows with 16-byte loads; three-term form with an
+//     accumulator, or the Clenshaw form (four vector passes instead of five).
+//   * k_blk_bal (epilogues EpiCheb / EpiClen) and k_kpm_bal : the same steps
+//     for matrices with uneven rows, parallel over (nonzero, 16-byte piece)
+//     with the products staged in LDS and summed per row in column order.
+//   * k_cheb_blk : B columns with one lane per (row, column).
+//   * k_kpm_blk : one degree of the KPM recurrence (SRC/kpmdos.c:129-156) for
+//     all random vectors at once, fused with the per-column dot product
 //     against v0 (block partial sums, deterministic second stage).
-//   * lan_* : Lanczos vector updates and classical Gram-Schmidt with the DGKS
-//     test (SRC/cheblanNr.c:219-246, SRC/miscla.c:211-240) as batched
-//     tall-skinny GEMV-T / GEMV-N over chunked Lanczos bases.
-// Support: Sturm-bisection restricted trace of the Lanczos tridiagonal
-// matrix, the closed-form DOS integral of the slicer, random fills.
+//   * k_lan_* : Lanczos vector updates and classical Gram-Schmidt with the
+//     DGKS test (SRC/cheblanNr.c:219-246, SRC/miscla.c:211-240) as batched
+//     tall-skinny GEMV-T / GEMV-N over chunked Lanczos bases (FP64 or FP32
+//     storage, FP64 sums).
+// Filter launches map blockIdx.x to row blocks XCD by XCD (xcd_block), so
+// that each of the 8 XCDs, with its own L2, owns one contiguous range of rows.
+// Support: Sturm-multisection restricted trace of the Lanczos tridiagonal
+// matrix, the closed-form DOS integral of the slicer, random fills, the
+// vector part of a generalized filter step (k_cheb_update), a plain copy
+// kernel (the bandwidth and counter reference).
 //
 // Every reduction is two-stage with a fixed order, so results are
 // deterministic for a given launch shape.  The filter kernels disable FMA
