@@ -13,16 +13,21 @@
 //
 // GPU mapping:
 //   * all slices of a batch step in lockstep; their current Lanczos vectors
-//     are the columns of one interleaved block, filtered together by a block
-//     SpMM with per-column coefficients and degrees (kernels.hip
-//     k_cheb_blk); a single slice uses the one-column kernel (k_cheb_vec);
+//     are packed, in groups of similar degree, into interleaved blocks that
+//     a block SpMM filters with per-column coefficients and degrees
+//     (kernels.hip: the row kernels k_cheb_blkrow / k_clen_blkrow, the
+//     nnz-balanced k_blk_bal for uneven rows); a single slice uses the
+//     one-column CSR-stream kernel (k_cheb_stream) or, when tuned, k_cheb_vec;
+//   * on the mixed path the filter runs in FP32 and the basis may be stored
+//     in FP32; the Ritz stage then refines the selected vectors by an FP64
+//     Rayleigh-Ritz projection (MFMA DGEMMs + rocSOLVER SYGVD);
 //   * the Lanczos bases live in HBM in chunks of `chunk_cols` columns that
 //     are allocated as the basis grows, so memory follows the actual step
 //     count rather than maxit;
 //   * alpha/beta, the DGKS decision and the norms stay on the device; the
 //     host reads two scalars per slice per step for the breakdown test and
 //     the tridiagonal matrix;
-//   * the restricted trace uses GPU Sturm bisection on T_k; the final
+//   * the restricted trace uses GPU Sturm multisection on T_k; the final
 //     eigenvectors of T come from rocSOLVER (divide and conquer), the Ritz
 //     vectors from rocBLAS DGEMM over the basis chunks, the residuals from the
 //     SpMV and column kernels.
