@@ -3,7 +3,9 @@
 // pEVSL_Kpmdos (SRC/kpmdos.c:36-197): all nvec random vectors advance
 // together as the columns of one interleaved block; each degree is one fused
 // launch (kernels.hip k_kpm_blk: SpMM + three-term recurrence + per-column
-// dot with v0).  The per-(vector, degree) dots come back to the host, where
+// dot with v0; k_kpm_bal, in blocks of at most 32 vectors, when a row has
+// more than 64 entries).  For the generalized problem the moments come from
+// the B-solve form (gen.cpp).  The per-(vector, degree) dots come back to the host, where
 // mu and the eigenvalue count are accumulated in the reference's order
 // (vector-major), so the host arithmetic matches the reference exactly.
 //

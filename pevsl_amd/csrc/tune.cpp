@@ -1,8 +1,11 @@
 // tune.cpp -- autotuner of the fused filter step and reorthogonalization
 // timing (pevsl_amd.h: pEVSL_Autotune, pEVSL_BenchReorth).  No counterpart
-// in the reference: it picks the lanes per row of the one-column kernel and
-// the block width of the batched SpMM for the matrix at hand, from HIP-event
-// timings of back-to-back launches on the instance's stream.
+// in the reference: it picks the lanes per row (or the CSR-stream kernel) of
+// the one-column filter, and the block width and block kernel (lane per
+// (row, column), row per thread, Clenshaw, nnz-balanced) of the batched SpMM,
+// for the matrix at hand, from HIP-event timings of back-to-back launches on
+// the instance's stream.  pEVSL_BenchBlock, pEVSL_BenchCopy and
+// pEVSL_BenchReorth time single kernels for profiling.
 #include <algorithm>
 #include <cmath>
 
