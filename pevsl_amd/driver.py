@@ -90,7 +90,10 @@ def read_matrix_market(path, idxin=1):
     (int32/int32/float64, columns sorted), with the reference reader's rules
     (TESTS/COMMON/parmm.c:11-162): indices start at idxin (1 for MM1, 0 for
     MM0); a 'symmetric' file stores one triangle and every off-diagonal
-    entry is mirrored.  Duplicate entries are summed."""
+    entry is mirrored.  Duplicate entries are summed.  As in the reference,
+    any positive idxin means 1-based.  Indices are read as exact integers
+    (the reference parses them through a float, exact only below 2^24)."""
+    idxin = 1 if idxin > 0 else 0
     with open(path, "r") as f:
         banner = f.readline().split()
         if len(banner) < 5 or banner[0] != "%%MatrixMarket" or banner[1].lower() != "matrix":

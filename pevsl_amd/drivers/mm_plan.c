@@ -19,8 +19,10 @@
 #include "pevsl.h"
 
 /* COO triplets of a sparse real/integer coordinate Matrix Market file;
- * returns 0 on success */
+ * returns 0 on success, 1-4 for an unreadable or unsupported file, 5 for an
+ * index outside the declared size */
 static int read_mm(const char *fn, int idxin, pevsl_Coo *coo) {
+  idxin = idxin > 0; /* any positive value means 1-based, as parmm.c:17 */
   FILE *f = fopen(fn, "r");
   if (!f) return 1;
   char line[1024], obj[64], fmt[64], field[64], symm[64];
@@ -46,6 +48,10 @@ static int read_mm(const char *fn, int idxin, pevsl_Coo *coo) {
       return 4;
     }
     const int ri = (int)r - idxin, ci = (int)c - idxin;
+    if (ri < 0 || ri >= nrow || ci < 0 || ci >= ncol) {
+      fclose(f);
+      return 5; /* index outside the declared size */
+    }
     coo->ir[m] = ri;
     coo->jc[m] = ci;
     coo->vv[m++] = v;

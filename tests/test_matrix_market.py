@@ -30,6 +30,9 @@ def test_symmetric_file_is_mirrored_and_mm0_mm1_indexing(tmp_path):
         .replace("3 2 -0.5", "2 1 -0.5").replace("3 3 4.0", "2 2 4.0")
     ia0, ja0, a0 = driver.read_matrix_market(_write(tmp_path, text0), 0)
     assert np.array_equal(ia, ia0) and np.array_equal(ja, ja0) and np.array_equal(a, a0)
+    # any positive idxin means 1-based (parmm.c: idxin = idxin > 0)
+    ia2, ja2, a2 = driver.read_matrix_market(_write(tmp_path, text), 2)
+    assert np.array_equal(ia, ia2) and np.array_equal(ja, ja2) and np.array_equal(a, a2)
 
 
 def test_general_file_is_not_mirrored(tmp_path):
@@ -59,3 +62,10 @@ def test_stiff1_fixture():
     o, ot = np.argsort(key), np.argsort(keyt)
     assert np.array_equal(key[o], keyt[ot]) and np.array_equal(a[o], a[ot])
     assert len(g["eigs_0.10_0.14"]) == 312
+
+
+def test_index_outside_declared_size_is_an_error(tmp_path):
+    text = "%%MatrixMarket matrix coordinate real general\n2 2 2\n1 1 1.0\n3 1 2.0\n"
+    import pytest
+    with pytest.raises(ValueError):
+        driver.read_matrix_market(_write(tmp_path, text), 1)
