@@ -155,7 +155,9 @@ void pEVSL_ParvecsGemtvWithWspace(double alp, pevsl_Parvecs *A, int nvecs,
 void pEVSL_ParvecsGemtv(double alp, pevsl_Parvecs *A, int nvecs,
                         pevsl_Parvec *x, double bet, double *y);
 
-/* ---- spmat.c (INC/pevsl.h:108-116): host sparse utilities --------------- */
+/* ---- spmat.c (INC/pevsl.h:108-116): host sparse utilities ---------------
+ * CooToCsr returns -1, and leaves csr untouched, when an index lies outside
+ * the declared size. */
 void pEVSL_CsrResize(int nrow, int ncol, int nnz, pevsl_Csr *csr);
 void pEVSL_FreeCsr(pevsl_Csr *csr);
 void pEVSL_CooResize(int nrow, int ncol, int nnz, pevsl_Coo *coo);

@@ -692,6 +692,11 @@ void pEVSL_FreeCoo(pevsl_Coo *coo) {
 int pEVSL_CooToCsr(int cooidx, pevsl_Coo *coo, pevsl_Csr *csr) {
   cooidx = cooidx != 0;
   const int nnz = coo->nnz, nr = coo->nrows;
+  // an index outside the declared size would be counted out of bounds
+  for (int i = 0; i < nnz; i++) {
+    const int r = coo->ir[i] - cooidx, c = coo->jc[i] - cooidx;
+    if (r < 0 || r >= nr || c < 0 || c >= coo->ncols) return -1;
+  }
   pEVSL_CsrResize(nr, coo->ncols, nnz, csr);
   std::vector<int> next((size_t)nr + 1, 0);
   for (int i = 0; i < nnz; i++) next[(size_t)(coo->ir[i] - cooidx) + 1]++;

@@ -281,3 +281,21 @@ def test_findpol_random_intervals_match_oracle(lib, damping, thresh):
             assert abs(pol.gam - opol.gam) <= 1e-13, intv
             np.testing.assert_allclose(driver.pol_mu(pol), opol.mu, rtol=1e-12, atol=1e-15)
         lib.pEVSL_FreePol(C.byref(pol))
+
+
+def test_coo_to_csr_rejects_out_of_range_indices(lib):
+    """An index outside the declared size returns -1 before anything is
+    counted or allocated (the reference's loop would index out of bounds)."""
+    class Coo(C.Structure):
+        _fields_ = [("nrows", C.c_int), ("ncols", C.c_int), ("nnz", C.c_int),
+                    ("ir", capi.c_int_p), ("jc", capi.c_int_p), ("vv", capi.c_double_p)]
+    for ir, jc, idx in (([0, 3], [0, 1], 0), ([0, 1], [0, 5], 0), ([-1, 1], [0, 1], 0),
+                        ([1, 4], [1, 2], 1), ([0, 2], [1, 2], 1)):
+        ir = np.array(ir, dtype=np.int32)
+        jc = np.array(jc, dtype=np.int32)
+        vv = np.ones(2)
+        coo = Coo(3, 3, 2, ir.ctypes.data_as(capi.c_int_p), jc.ctypes.data_as(capi.c_int_p),
+                  vv.ctypes.data_as(capi.c_double_p))
+        csr = capi.Csr()
+        assert lib.pEVSL_CooToCsr(idx, C.byref(coo), C.byref(csr)) == -1, (ir, jc, idx)
+        assert not csr.ia
