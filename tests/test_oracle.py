@@ -217,6 +217,25 @@ def test_kpm_moments_against_spectral_sum():
     assert abs(mu[0] - 1.0 / np.pi) < 1e-15
 
 
+@pytest.mark.parametrize("a,b,damping", [(2.0, 4.0, 1), (0.0, 12.0, 1), (5.5, 5.6, 1),
+                                         (-1.0, 3.0, 1), (2.0, 4.0, 0), (2.0, 4.0, 2)])
+def test_kpm_count_is_the_integral_of_its_dos(a, b, damping):
+    """The count the KPM returns (accumulated from sine terms during the
+    recurrence, SRC/kpmdos.c:127,155,182) equals n times the integral of the
+    DOS its own moments define over [a, b] clipped to the frame, computed by
+    the independently pinned intchx quadrature.  A dropped term, a wrong sign
+    or a wrong damping factor in either formula fails this."""
+    lam = np.linspace(0.05, 11.95, 200)
+    n = len(lam)
+    A = orc.Csr(np.arange(n + 1), np.arange(n), lam)
+    orc.srand(7)
+    rc, mu, ecnt = orc.kpmdos(A, 40, damping, 5, [a, b, 0.0, 12.0])
+    assert rc == 0
+    xa, xb = (max(a, 0.0) - 6.0) / 6.0, (min(b, 12.0) - 6.0) / 6.0
+    y = orc.intchx(mu, np.array([xa, xb]))
+    assert abs(ecnt - n * y[1]) <= 1e-8 * max(1.0, abs(ecnt))
+
+
 def test_kpm_count_c1_within_stochastic_error():
     A = orc.Csr(lap=(20, 20, 20))
     orc.srand(0)
