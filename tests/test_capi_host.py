@@ -299,3 +299,12 @@ def test_coo_to_csr_rejects_out_of_range_indices(lib):
         csr = capi.Csr()
         assert lib.pEVSL_CooToCsr(idx, C.byref(coo), C.byref(csr)) == -1, (ir, jc, idx)
         assert not csr.ia
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """No fallback: loading from a path without the library raises with the
+    build command in the message."""
+    import pytest
+    with pytest.raises(RuntimeError) as e:
+        capi.load(str(tmp_path / "libpevsl_amd.so"))
+    assert "not found" in str(e.value) and "build" in str(e.value)
