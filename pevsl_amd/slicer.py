@@ -418,6 +418,15 @@ def main(argv=None):
     ap.add_argument("--nmin", type=int, default=100000)
     ap.add_argument("--nmax", type=int, default=500000)
     args = ap.parse_args(argv)
+    for name in ("nmat", "nint", "epochs", "batch"):
+        if getattr(args, name) < 1:
+            ap.error("--%s must be at least 1" % name)
+    if args.nint_irregular is not None and args.nint_irregular < 1:
+        ap.error("--nint-irregular must be at least 1")
+    if not 0.0 < args.holdout < 1.0:
+        ap.error("--holdout must lie in (0, 1)")
+    if not 1 <= args.nmin <= args.nmax:
+        ap.error("--nmin and --nmax must satisfy 1 <= nmin <= nmax")
     lib = capi.load()
     t0 = time.time()
     files = args.moments.split(",") if args.moments else []

@@ -16,3 +16,14 @@ def test_unknown_mode_is_rejected():
     with pytest.raises(SystemExit) as e:
         lap_slicing.main(["--mode", "fp16"])
     assert e.value.code == 2
+
+
+@pytest.mark.parametrize("args", [["--nmat", "-5", "--gen-only"], ["--nint", "0"],
+                                  ["--epochs", "0"], ["--holdout", "1.5"],
+                                  ["--nmin", "10", "--nmax", "5"]])
+def test_slicer_cli_rejects_bad_sizes(args):
+    """The training CLI checks its sizes before loading the library."""
+    from pevsl_amd import slicer
+    with pytest.raises(SystemExit) as e:
+        slicer.main(args)
+    assert e.value.code == 2
