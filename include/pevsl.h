@@ -40,7 +40,9 @@ void pEVSL_FreePol(pevsl_Polparams *pol);
 /* ---- kpmdos.c (INC/pevsl.h:17-18) ----------------------------------------
  * Chebyshev moments mu[0..Mdeg] (caller-allocated) of the spectral density
  * and the eigenvalue-count estimate *ecnt for [intv[0], intv[1]].  All nvec
- * random vectors advance together as columns of one block SpMM on the GPU.
+ * random vectors advance together as columns of one block SpMM on the GPU;
+ * with an operator installed by pEVSL_SetAMatvec (host callback) they run
+ * one after the other, one callback per degree, as the reference's loop.
  * ngroups/groupid/gl_comm are accepted for source compatibility; with one
  * process ngroups must be 1.                           SRC/kpmdos.c:36-197 */
 /* INC/pevsl.h:17 */

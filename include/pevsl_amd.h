@@ -28,6 +28,9 @@ typedef struct {
   long long nmatvec;/* products with A (filter + Ritz residuals)            */
   double tr;        /* restricted trace at the last test                    */
   double t_total;   /* wall seconds of the solve                            */
+  double tol_accept;/* residual threshold applied to the Ritz pairs: the
+                       caller's tol, or on the FP32-filter path
+                       max(tol, mixed_tol * rho) (pEVSL_SetMixedTol)        */
 } pevsl_LanInfo;
 
 /* Accumulated device time (seconds, HIP events) by phase, and algorithmic
@@ -69,7 +72,10 @@ int pEVSL_SetBasisPrecision(pevsl_Data *pevsl, int prec);
  * Laplacian's [0, 12]).  Ritz vectors built from an FP32-filtered Krylov
  * basis carry residuals of about 1e-7 rho, while their FP64 Rayleigh-Ritz
  * values stay accurate to ~1e-12 relative; eigenvalue errors are reported
- * separately. */
+ * separately.  On the FP32-filter path this threshold therefore replaces a
+ * smaller tol passed to pEVSL_ChebLanNr / pEVSL_ChebLanNrBatch; the value
+ * applied is reported per slice in pevsl_LanInfo.tol_accept.  The FP64 path
+ * always applies the caller's tol. */
 int pEVSL_SetMixedTol(pevsl_Data *pevsl, double tol);
 
 /* Ritz stage: 0 = the reference's per-vector Ritz pairs (SRC/cheblanNr.c:
@@ -160,6 +166,19 @@ int pEVSL_ChebAv(pevsl_Data *pevsl, pevsl_Polparams *pol, pevsl_Parvec *v,
 int pEVSL_GetLanInfo(pevsl_Data *pevsl, int s, pevsl_LanInfo *info);
 int pEVSL_GetKernelStats(pevsl_Data *pevsl, pevsl_KernelStats *st);
 
+/* Live timing of the hot kernels: HIP events on the instance's stream around
+ * each run of back-to-back launches of one kernel (the filter steps of a
+ * block or column, a Gram-Schmidt pass), accumulated per kernel since the
+ * last pEVSL_StatsReset: launches, seconds, algorithmic bytes.  Copies up
+ * to maxn entries to out and sets *count to the number recorded. */
+typedef struct {
+  char name[48];      /* e.g. "k_clen_blkrow<float,8>"                     */
+  long long launches;
+  double seconds;
+  double bytes;
+} pevsl_KernelTime;
+int pEVSL_GetKernelTimes(pevsl_Data *pevsl, int maxn, pevsl_KernelTime *out, int *count);
+
 /* y = A x with device pointers (HBM), on the instance's stream; blocks
  * until done. */
 int pEVSL_DeviceMatvec(pevsl_Data *pevsl, const double *d_x, double *d_y);
@@ -199,6 +218,35 @@ int pEVSL_Autotune(pevsl_Data *pevsl, int reps, pevsl_TuneResult *res);
  * (pEVSL_SetBlockKernel; Clenshaw forms need the FP32 filter): seconds per
  * step and its algorithmic bytes.  -1 on invalid arguments. */
 int pEVSL_BenchBlock(pevsl_Data *pevsl, int B, int reps, double *sec, double *bytes);
+
+/* Times `reps` steps of a batched filtered-Lanczos solve at a fixed basis
+ * size k, after `warmup` untimed ones: every slice s (interval intvs[4s..],
+ * filter pols[s]) holds k + 2 basis columns, and one step runs what one
+ * iteration of pEVSL_ChebLanNrBatch runs -- the filter groups (block SpMM
+ * steps at the slices' own degrees), the three-term update and the
+ * Gram-Schmidt pass against k + 1 columns -- and its per-step
+ * synchronization.  The instance's kernel statistics and per-kernel timers
+ * (pEVSL_GetKernelTimes) are reset after the warm-up, so they cover the
+ * timed steps.  With graph_reps > 0 the first filter group's launch sequence
+ * is also captured in a HIP graph and replayed graph_reps times against as
+ * many eager calls.  Needs the device matrix (pEVSL_SetAParcsr); -1 on
+ * invalid arguments.  No counterpart in the reference. */
+typedef struct {
+  int nslices, k, reps;
+  double sec_step;       /* HIP-event time of one step                      */
+  double wall_step;      /* host wall time of one step, synchronization incl. */
+  double sec_filter;     /* per step: filter groups (pack, steps, unpack)    */
+  double sec_reorth;     /* per step: three-term update + Gram-Schmidt       */
+  double sec_cgs;        /* per step: Gram-Schmidt passes                    */
+  double filter_bytes;   /* algorithmic bytes per step                       */
+  double cgs_bytes;
+  long long filter_launches;  /* block/column filter launches per step       */
+  double eager_group_sec;     /* first group: one eager call (wall)          */
+  double graph_group_sec;     /* the same sequence as one graph replay (wall) */
+  int graph_nodes, graph_group_width;
+} pevsl_StepBench;
+int pEVSL_BenchBatchStep(pevsl_Data *pevsl, int nslices, double *intvs, pevsl_Polparams *pols,
+                         int k, int warmup, int reps, int graph_reps, pevsl_StepBench *out);
 
 /* Times `reps` launches of a plain copy kernel over n doubles (read n,
  * write n; 16-byte accesses) on the current device: seconds per launch and

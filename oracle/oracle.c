@@ -424,7 +424,10 @@ int orc_spslicer_kpm(double *sli, const double *mu, int Mdeg,
   while (++ls < n_int) {
     while (ii < npts && yi[ii] < target) ii++;
     if (ii == npts) break;
-    if ((target - yi[ii - 1]) < (yi[ii] - target)) ii--;
+    /* SRC/spslicer.c:116-118 reads yi[ii-1] even at ii = 0 (a target <= 0,
+     * when the DOS integral over the interval is not positive); the grid's
+     * first point is then the cut */
+    if (ii > 0 && (target - yi[ii - 1]) < (yi[ii] - target)) ii--;
     sli[ls] = ctr + wid * xi[ii];
     target = yi[ii] + (yi[npts - 1] - yi[ii]) / (n_int - ls);
   }

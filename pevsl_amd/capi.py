@@ -85,7 +85,12 @@ class LanInfo(C.Structure):
     _fields_ = [("its", C.c_int), ("kdim", C.c_int), ("nbreak", C.c_int),
                 ("nconv", C.c_int), ("nev", C.c_int), ("deg", C.c_int),
                 ("nmatvec", C.c_longlong), ("tr", C.c_double),
-                ("t_total", C.c_double)]
+                ("t_total", C.c_double), ("tol_accept", C.c_double)]
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("launches", C.c_longlong),
+                ("seconds", C.c_double), ("bytes", C.c_double)]
 
 
 class TuneResult(C.Structure):
@@ -96,6 +101,16 @@ class TuneResult(C.Structure):
                 ("block_rowkernel", C.c_int), ("t_blockclen", C.c_double * 6),
                 ("bytes_clen", C.c_double * 6), ("t_blockbal", C.c_double * 6),
                 ("t_blockbalclen", C.c_double * 6)]
+
+
+class StepBench(C.Structure):
+    _fields_ = [("nslices", C.c_int), ("k", C.c_int), ("reps", C.c_int),
+                ("sec_step", C.c_double), ("wall_step", C.c_double),
+                ("sec_filter", C.c_double), ("sec_reorth", C.c_double),
+                ("sec_cgs", C.c_double), ("filter_bytes", C.c_double),
+                ("cgs_bytes", C.c_double), ("filter_launches", C.c_longlong),
+                ("eager_group_sec", C.c_double), ("graph_group_sec", C.c_double),
+                ("graph_nodes", C.c_int), ("graph_group_width", C.c_int)]
 
 
 class KernelStats(C.Structure):
@@ -227,6 +242,9 @@ SIGNATURES = {
     "pEVSL_SetBlockKernel": (C.c_int, [DataP, C.c_int]),
     "pEVSL_RandSeed": (None, [C.c_uint]),
     "pEVSL_BenchBlock": (C.c_int, [DataP, C.c_int, C.c_int, c_double_p, c_double_p]),
+    "pEVSL_BenchBatchStep": (C.c_int, [DataP, C.c_int, c_double_p, C.POINTER(Polparams),
+                                       C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(StepBench)]),
     "pEVSL_ChebIterSetup": (C.c_int, [C.c_double, C.c_double, C.c_int, C.POINTER(Parcsr),
                                       C.POINTER(C.c_void_p)]),
     "pEVSL_ChebIterSolv1": (None, [c_double_p, c_double_p, C.c_void_p]),
@@ -288,6 +306,8 @@ SIGNATURES = {
                                C.POINTER(Parvec), C.POINTER(Parvec)]),
     "pEVSL_GetLanInfo": (C.c_int, [DataP, C.c_int, C.POINTER(LanInfo)]),
     "pEVSL_GetKernelStats": (C.c_int, [DataP, C.POINTER(KernelStats)]),
+    "pEVSL_GetKernelTimes": (C.c_int, [DataP, C.c_int, C.POINTER(KernelTime),
+                                       C.POINTER(C.c_int)]),
     "pEVSL_DeviceMatvec": (C.c_int, [DataP, C.c_void_p, C.c_void_p]),
     "pEVSL_DeviceInfo": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_size_t)]),
     "pEVSL_SymmTridEigvals": (C.c_int, [C.c_int, c_double_p, c_double_p, C.c_double,

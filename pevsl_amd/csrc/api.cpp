@@ -45,6 +45,81 @@ const DevCsr *dev_matrix_of(pevsl_Parcsr *A) {
   return &static_cast<DevMatrix *>(A->dev)->csr;
 }
 
+KernTimers::~KernTimers() {
+  for (Pend &q : pend) {
+    (void)hipEventDestroy(q.a);
+    (void)hipEventDestroy(q.b);
+  }
+  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+hipEvent_t KernTimers::take() {
+  if (pool.empty()) {
+    hipEvent_t e;
+    PV_HIP(hipEventCreate(&e));
+    return e;
+  }
+  hipEvent_t e = pool.back();
+  pool.pop_back();
+  return e;
+}
+
+long KernTimers::begin(hipStream_t s, const char *name) {
+  // bound the events in flight: resolve the recorded runs first
+  if (pend.size() >= 256) flush();
+  size_t i = 0;
+  while (i < acc.size() && acc[i].name != name) i++;
+  if (i == acc.size()) {
+    acc.emplace_back();
+    acc.back().name = name;
+  }
+  Pend q{i, take(), take(), -1, 0.0};
+  PV_HIP(hipEventRecord(q.a, s));
+  pend.push_back(q);
+  return (long)pend.size() - 1;
+}
+
+void KernTimers::end(hipStream_t s, long tok, long long launches, double bytes) {
+  Pend &q = pend[(size_t)tok];
+  q.launches = launches;
+  q.bytes = bytes;
+  PV_HIP(hipEventRecord(q.b, s));
+}
+
+void KernTimers::flush() {
+  std::vector<Pend> open;  // begun, end not yet recorded
+  for (Pend &q : pend) {
+    if (q.launches < 0) {
+      open.push_back(q);
+      continue;
+    }
+    PV_HIP(hipEventSynchronize(q.b));
+    float ms = 0.f;
+    PV_HIP(hipEventElapsedTime(&ms, q.a, q.b));
+    Acc &a = acc[q.acc];
+    a.launches += q.launches;
+    a.sec += 1e-3 * ms;
+    a.bytes += q.bytes;
+    pool.push_back(q.a);
+    pool.push_back(q.b);
+  }
+  pend.swap(open);
+}
+
+void KernTimers::reset() {
+  flush();
+  acc.clear();
+}
+
+double KernTimers::seconds_matching(const char *prefix) {
+  flush();
+  double t = 0.0;
+  const size_t L = strlen(prefix);
+  for (const Acc &a : acc)
+    if (a.name.compare(0, L, prefix) == 0) t += a.sec;
+  return t;
+}
+
 // about three nonzeros per lane (measured best for 7-point stencils: 2)
 int auto_lanes(double mean_row) {
   int L = 1;
@@ -53,6 +128,7 @@ int auto_lanes(double mean_row) {
 }
 
 void apply_A(Ctx &c, const double *dx, double *dy) {
+  if (c.stats) c.stats->n_mvA++;
   if (c.A) {
     const int lanes =
         c.lanes ? c.lanes
@@ -66,7 +142,9 @@ void apply_A(Ctx &c, const double *dx, double *dy) {
   PV_HIP(hipMemcpyAsync(c.h_x.data(), dx, n * sizeof(double),
                         hipMemcpyDeviceToHost, c.stream));
   PV_HIP(hipStreamSynchronize(c.stream));
+  const double t0 = wtime();
   c.host_mv(c.h_x.data(), c.h_y.data(), c.host_mv_data);
+  if (c.stats) c.stats->t_mvA += wtime() - t0;
   PV_HIP(hipMemcpyAsync(dy, c.h_y.data(), n * sizeof(double),
                         hipMemcpyHostToDevice, c.stream));
 }
@@ -171,6 +249,7 @@ int pEVSL_Start(MPI_Comm comm, pevsl_Data **data) {
   PV_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   PV_BLAS(rocblas_create_handle(&c->blas));
   PV_BLAS(rocblas_set_stream(c->blas, c->stream));
+  c->stats = p->stats;
   p->dev = c;
   pEVSL_StatsReset(p);
   *data = p;
@@ -316,6 +395,22 @@ int pEVSL_GetLanInfo(pevsl_Data *p, int s, pevsl_LanInfo *info) {
 
 int pEVSL_GetKernelStats(pevsl_Data *p, pevsl_KernelStats *st) {
   *st = ctx_of(p).kstats;
+  return 0;
+}
+
+int pEVSL_GetKernelTimes(pevsl_Data *p, int maxn, pevsl_KernelTime *out, int *count) {
+  Ctx &c = ctx_of(p);
+  c.kt.flush();
+  *count = (int)c.kt.acc.size();
+  if (maxn < 0 || (maxn > 0 && !out)) return -1;
+  for (int i = 0; i < std::min(maxn, *count); i++) {
+    const KernTimers::Acc &a = c.kt.acc[(size_t)i];
+    memset(out[i].name, 0, sizeof(out[i].name));
+    strncpy(out[i].name, a.name.c_str(), sizeof(out[i].name) - 1);
+    out[i].launches = a.launches;
+    out[i].seconds = a.sec;
+    out[i].bytes = a.bytes;
+  }
   return 0;
 }
 
@@ -840,7 +935,10 @@ double pEVSL_Wtime(void) { return wtime(); }
 
 void pEVSL_StatsReset(pevsl_Data *p) {
   memset(p->stats, 0, sizeof(pevsl_Stat));
-  if (p->dev) memset(&ctx_of(p).kstats, 0, sizeof(pevsl_KernelStats));
+  if (p->dev) {
+    memset(&ctx_of(p).kstats, 0, sizeof(pevsl_KernelStats));
+    ctx_of(p).kt.reset();
+  }
 }
 
 void pEVSL_StatsPrint(pevsl_Data *p, FILE *f) {

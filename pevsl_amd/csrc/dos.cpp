@@ -143,6 +143,46 @@ void kpm_moments(Ctx &c, int n, int Mdeg, int nvec, double ctr, double wid,
   }
 }
 
+// KPM moments through an operator given as a host callback (pEVSL_SetAMatvec):
+// the reference's per-vector loop (SRC/kpmdos.c:102-157), one product per
+// degree through apply_A and the vector part of the recurrence on the device.
+void kpm_moments_op(Ctx &c, int n, int Mdeg, int nvec, double ctr, double wid,
+                    std::vector<double> &dots) {
+  const hipStream_t s = c.stream;
+  DBuf<double> buf, dd;
+  double *v = buf.ensure((size_t)4 * n), *vk = v + n, *vkm1 = v + 2 * (size_t)n,
+         *vkp1 = v + 3 * (size_t)n;
+  const int npb = dot_part_blocks(n);
+  double *pt = dd.ensure((size_t)Mdeg + npb + 8);
+  std::vector<double> hd((size_t)Mdeg + 1);
+  for (int m = 0; m < nvec; m++) {
+    rand_vectors(c, n, 1, v, (size_t)n);
+    const int nb0 = dot_partials(n, v, v, pt + Mdeg + 1, s);
+    reduce_partials(pt + Mdeg + 1, nb0, 1, pt + Mdeg, s);
+    PV_HIP(hipMemcpyAsync(hd.data() + Mdeg, pt + Mdeg, sizeof(double), hipMemcpyDeviceToHost,
+                          s));
+    PV_HIP(hipStreamSynchronize(s));
+    scale_vec(n, v, nullptr, 1.0 / sqrt(hd[(size_t)Mdeg]), s);
+    copy_f64((size_t)n, v, vk, s);
+    fill_zero_f64((size_t)n, vkm1, s);
+    for (int k = 0; k < Mdeg; k++) {
+      apply_A(c, vk, vkp1);
+      const double scal = (k == 0 ? 1.0 : 2.0) / wid;
+      cheb_update(n, vk, vkm1, vkp1, nullptr, ctr, scal, 0.0, s);
+      double *t = vkm1;
+      vkm1 = vk;
+      vk = vkp1;
+      vkp1 = t;
+      const int nb = dot_partials(n, vk, v, pt + Mdeg + 1, s);
+      reduce_partials(pt + Mdeg + 1, nb, 1, pt + k, s);
+      c.kstats.filter_steps++;
+    }
+    PV_HIP(hipMemcpyAsync(hd.data(), pt, sizeof(double) * Mdeg, hipMemcpyDeviceToHost, s));
+    PV_HIP(hipStreamSynchronize(s));
+    std::copy(hd.begin(), hd.begin() + Mdeg, dots.begin() + (long)m * Mdeg);
+  }
+}
+
 }  // namespace
 }  // namespace pevsl
 
@@ -158,7 +198,7 @@ int pEVSL_Kpmdos(pevsl_Data *pevsl, int Mdeg, int damping, int nvec, double *int
   Ctx &c = ctx_of(pevsl);
   if (ngroups > 1) return PEVSL_NOT_IMPLEMENT;  // one process: one group
   if (check_intv(intv, stdout) < 0) return -1;
-  if (!c.A && !pevsl->ifGenEv) return PEVSL_NOT_IMPLEMENT;  // block KPM needs the device matrix
+  if (!c.A && !c.host_mv && !pevsl->ifGenEv) return -3;  // no operator installed
   const int n = pevsl->n;
   const double aa = std::max(intv[0], intv[2]), bb = std::min(intv[1], intv[3]);
   if (intv[0] < intv[2] || intv[1] > intv[3])
@@ -175,14 +215,22 @@ int pEVSL_Kpmdos(pevsl_Data *pevsl, int Mdeg, int damping, int nvec, double *int
   std::vector<double> dots((size_t)nvec * Mdeg);
   EvTimer ev;
   ev.start(c.stream);
-  if (pevsl->ifGenEv)
+  if (pevsl->ifGenEv) {
     kpm_moments_gen(c, pevsl, Mdeg, nvec, ctr, wid, dots);
-  else if (c.filter_prec == PEVSL_PREC_FP32)
+  } else if (!c.A) {
+    // host-callback operator: FP64 vectors whatever the filter precision
+    if ((int)c.h_x.size() != n) {
+      c.h_x.assign((size_t)n, 0.0);
+      c.h_y.assign((size_t)n, 0.0);
+    }
+    kpm_moments_op(c, n, Mdeg, nvec, ctr, wid, dots);
+  } else if (c.filter_prec == PEVSL_PREC_FP32)
     kpm_moments<float>(c, n, Mdeg, nvec, ctr, wid, dots);
   else
     kpm_moments<double>(c, n, Mdeg, nvec, ctr, wid, dots);
   ev.stop(c.stream);
   c.kstats.t_kpm += ev.seconds();
+  if (c.A && !pevsl->ifGenEv) pevsl->stats->n_mvA += (size_t)nvec * Mdeg;
   // accumulation in the reference order (SRC/kpmdos.c:104-157)
   memset(mu, 0, sizeof(double) * (size_t)(Mdeg + 1));
   double tcnt = 0.0;
@@ -242,7 +290,10 @@ int pEVSL_SpslicerKpm(double *sli, double *mu, int Mdeg, double *intv, int n_int
   while (++ls < n_int) {
     while (ii < npts && yi[(size_t)ii] < target) ii++;
     if (ii == npts) break;
-    if ((target - yi[(size_t)ii - 1]) < (yi[(size_t)ii] - target)) ii--;
+    // best of the two grid points around the target; at ii = 0 (target <= 0:
+    // the DOS integral over the interval is not positive) the first point,
+    // where SRC/spslicer.c:116-118 would read yi[-1]
+    if (ii > 0 && (target - yi[(size_t)ii - 1]) < (yi[(size_t)ii] - target)) ii--;
     sli[ls] = ctr + wid * xi[(size_t)ii];
     target = yi[(size_t)ii] + (yi[(size_t)npts - 1] - yi[(size_t)ii]) / (n_int - ls);
   }

@@ -68,6 +68,42 @@ struct DevMatrix {
   DBuf<float> a32;
 };
 
+// Live per-kernel timing on an instance's stream: HIP events around a run of
+// back-to-back launches of one kernel (nothing else enqueued in between),
+// resolved lazily once the events have completed.  Accumulates launches,
+// seconds and algorithmic bytes per kernel name (pEVSL_GetKernelTimes).
+struct KernTimers {
+  struct Acc {
+    std::string name;
+    long long launches = 0;
+    double sec = 0.0, bytes = 0.0;
+  };
+  struct Pend {
+    size_t acc;
+    hipEvent_t a, b;
+    long long launches;
+    double bytes;
+  };
+  std::vector<Acc> acc;
+  std::vector<Pend> pend;
+  std::vector<hipEvent_t> pool;
+  KernTimers() = default;
+  KernTimers(const KernTimers &) = delete;
+  KernTimers &operator=(const KernTimers &) = delete;
+  ~KernTimers();
+  // records the start event; returns a token for end()
+  long begin(hipStream_t s, const char *name);
+  // records the end event of the run of `launches` launches
+  void end(hipStream_t s, long tok, long long launches, double bytes);
+  // waits for the pending runs and accumulates them
+  void flush();
+  void reset();
+  double seconds_matching(const char *prefix);
+
+ private:
+  hipEvent_t take();
+};
+
 // Rows longer than this make the batched filter default to the nnz-balanced
 // block kernels.
 constexpr int kUnevenRow = 64;
@@ -95,6 +131,8 @@ struct Ctx {
   int chunk_cols = 512;            // Lanczos basis allocation granularity
   std::vector<pevsl_LanInfo> info; // per slice of the last (batched) solve
   pevsl_KernelStats kstats{};      // accumulated device timings
+  KernTimers kt;                   // per-kernel HIP-event timings
+  pevsl_Stat *stats = nullptr;     // the instance's pevsl_Stat (n_mvA, t_mvA)
   // staging for host callbacks / small copies
   DBuf<double> cb_x, cb_y;
   std::vector<double> h_x, h_y;

@@ -82,6 +82,27 @@ size_t filter_launch_bytes(const DevCsr &A, int elem, int ncol, bool first) {
                   (first ? 3.0 : 5.0) * vec * ncol);
 }
 
+// name of a filter kernel for the per-kernel timers
+std::string filter_kernel_name(bool f32, int kind, int B, bool stream, int lanes) {
+  const char *t = f32 ? "float" : "double";
+  char nm[48];
+  if (B == 0) {
+    if (stream)
+      snprintf(nm, sizeof nm, "k_cheb_stream<%s>", t);
+    else
+      snprintf(nm, sizeof nm, "k_cheb_vec<%s,%d>", t, lanes);
+  } else if (kind >= 3) {
+    snprintf(nm, sizeof nm, "k_blk_bal<%s,%d,%s>", t, B, kind == 4 ? "EpiClen" : "EpiCheb");
+  } else if (kind == 2) {
+    snprintf(nm, sizeof nm, "k_clen_blkrow<%s,%d>", t, B);
+  } else if (kind == 1) {
+    snprintf(nm, sizeof nm, "k_cheb_blkrow<%s,%d>", t, B);
+  } else {
+    snprintf(nm, sizeof nm, "k_cheb_blk<%s,%d>", t, B);
+  }
+  return nm;
+}
+
 }  // namespace
 
 // FP32 filter y = p((A - cc)/dd) v on FP32 columns; w holds 3n floats.
@@ -97,6 +118,9 @@ static void cheb_filter_f32(Ctx &c, const pevsl_Polparams *pol, const float *x32
   float *b[3] = {w, w + (size_t)n, w + 2 * (size_t)n};
   const float *vk = x32, *vkm1 = nullptr;
   int nb = 0;
+  const long tok = m > 0 ? c.kt.begin(s, filter_kernel_name(true, 0, 0, stream, lanes).c_str())
+                         : -1;
+  double bytes = 0.0;
   for (int k = 1; k <= m; k++) {
     float *vkp1 = b[nb];
     nb = (nb + 1) % 3;
@@ -108,8 +132,10 @@ static void cheb_filter_f32(Ctx &c, const pevsl_Polparams *pol, const float *x32
                            mu[k], k == 1, lanes, s);
     vkm1 = vk;
     vk = vkp1;
-    c.kstats.filter_bytes += (double)filter_launch_bytes(A, 4, 1, k == 1);
+    bytes += (double)filter_launch_bytes(A, 4, 1, k == 1);
   }
+  if (tok >= 0) c.kt.end(s, tok, m, bytes);
+  c.kstats.filter_bytes += bytes;
   c.kstats.filter_steps += m;
   c.kstats.filter_launches += m;
 }
@@ -152,6 +178,8 @@ void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
     double *b[3] = {w, w + n, w + 2 * (size_t)n};
     const double *vk = dv, *vkm1 = nullptr;
     int nb = 0;
+    const long tok = c.kt.begin(s, filter_kernel_name(false, 0, 0, stream, lanes).c_str());
+    const double bytes0 = c.kstats.filter_bytes;
     for (int k = 1; k <= m; k++) {
       double *vkp1 = b[nb];
       nb = (nb + 1) % 3;
@@ -166,6 +194,7 @@ void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
       vk = vkp1;
       c.kstats.filter_bytes += (double)filter_launch_bytes(A, 8, 1, k == 1);
     }
+    c.kt.end(s, tok, m, c.kstats.filter_bytes - bytes0);
   } else {
     float *w = reinterpret_cast<float *>(c.scratch64.ensure((size_t)5 * n / 2 + 8));
     float *x32 = w, *y32 = w + n;
@@ -222,6 +251,8 @@ class BatchSolver {
 
   int run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit,
           pevsl_Polparams *pols, FILE *fstats);
+  int bench(double *intvs, pevsl_Polparams *pols, int k, int warmup, int reps,
+            int graph_reps, pevsl_StepBench *out);
   void outputs(int *nevOut, double **lamo, pevsl_Parvecs **Wo, double **reso,
                pevsl_Parvec *vinit);
 
@@ -258,10 +289,15 @@ class BatchSolver {
   void ritz(int s, double tol);
   void ritz_rr(int s, double tol);
   void free_basis(int s);
+  int prepare(double *intvs, int *maxits, pevsl_Polparams *pols, double tall);
+
+  // the filter groups of one step: at most the tuned width, one interval map
+  std::vector<std::vector<int>> filter_groups(const std::vector<int> &act) const;
 
   Ctx &c_;
   pevsl_Data *p_;
   int ns_;
+  bool capture_ = false;  // filter_block enqueues kernels only (graph capture)
   int n_ = 0;
   bool b32_ = false;  // FP32 basis storage
   size_t vb_ = 8;     // basis element bytes
@@ -378,6 +414,8 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
   // host staging (mus_h_, cp_h_) is rewritten only after the per-step
   // synchronization, when earlier copies have completed
   if (key != mus_key_) {
+    if (capture_) fatal("filter_block: coefficients not staged before capture", __FILE__,
+                        __LINE__);
     PV_HIP(hipStreamSynchronize(c_.stream));
     std::vector<double> &mh = mus_h_;
     mh.assign((size_t)(degmax + 1) * B, 0.0);
@@ -390,17 +428,20 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
                           hipMemcpyHostToDevice, c_.stream));
     mus_key_ = key;
   }
-  // column pointers: inputs z_k, outputs z_{k+1}
-  PV_HIP(hipStreamSynchronize(c_.stream));
-  std::vector<void *> &cp = cp_h_;
-  cp.assign((size_t)2 * B, nullptr);
-  for (int ci = 0; ci < g; ci++) {
-    cp[(size_t)ci] = col(grp[(size_t)ci], k);
-    cp[(size_t)B + ci] = col(grp[(size_t)ci], k + 1);
+  // column pointers: inputs z_k, outputs z_{k+1} (under graph capture the
+  // pointers of this group and step are already on the device)
+  if (!capture_) {
+    PV_HIP(hipStreamSynchronize(c_.stream));
+    std::vector<void *> &cp = cp_h_;
+    cp.assign((size_t)2 * B, nullptr);
+    for (int ci = 0; ci < g; ci++) {
+      cp[(size_t)ci] = col(grp[(size_t)ci], k);
+      cp[(size_t)B + ci] = col(grp[(size_t)ci], k + 1);
+    }
+    colptr_.ensure(cp.size());
+    PV_HIP(hipMemcpyAsync(colptr_.p, cp.data(), sizeof(void *) * cp.size(),
+                          hipMemcpyHostToDevice, c_.stream));
   }
-  colptr_.ensure(cp.size());
-  PV_HIP(hipMemcpyAsync(colptr_.p, cp.data(), sizeof(void *) * cp.size(),
-                        hipMemcpyHostToDevice, c_.stream));
   const double cc = S_[(size_t)grp[0]].pol->cc, dd = S_[(size_t)grp[0]].pol->dd;
   std::vector<int> degs;
   for (int s : grp) degs.push_back(S_[(size_t)s].pol->deg);
@@ -430,13 +471,18 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
                            c_.stream);
     }
     T *yout = y;
+    const std::string kname = filter_kernel_name(f32, kind, B, false, 0);
     if (clen && degmax >= 1) {
       // Clenshaw sums from the top degree down: b_d = mu_d x + (2/dd)(A - cc)
       // b_{d+1} - b_{d+2}; the d = 0 step (1/dd) is the filtered vector.
       // Three rotating buffers; x (the input block) is read by every step.
+      // The top term (no product) is a column scaling; the degmax product
+      // steps after it are timed as one run of the block kernel.
       T *ring[3] = {y, b[0], b[1]};
       const T *b1 = nullptr, *b2 = nullptr;
       int nr = 0;
+      long tok = -1;
+      double kbytes = 0.0;
       for (int d = degmax; d >= 0; d--) {
         T *out = ring[nr];
         if ((const T *)out == b1 || (const T *)out == b2) {
@@ -444,6 +490,7 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
           out = ring[nr];
         }
         nr = (nr + 1) % 3;
+        if (b1 && tok < 0 && !capture_) tok = c_.kt.begin(c_.stream, kname.c_str());
         if (bal)
           clen_step_bal<T>(A, B, b1, b2, x, out, cc, d == 0 ? 1.0 / dd : 2.0 / dd, mus_.p, d,
                            c_.stream);
@@ -453,7 +500,9 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
         int nact = 0;  // columns whose series has reached degree d + 1
         while (nact < g && degs[(size_t)nact] >= d + 1) nact++;
         if (b1) {
-          c_.kstats.filter_bytes += (double)clen_launch_bytes(A, (int)elem, nact, b2 != nullptr);
+          const double lb = (double)clen_launch_bytes(A, (int)elem, nact, b2 != nullptr);
+          kbytes += lb;
+          c_.kstats.filter_bytes += lb;
           c_.kstats.filter_steps += nact;
           c_.kstats.filter_launches++;
         }
@@ -461,11 +510,14 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
         b1 = out;
         yout = out;
       }
+      if (tok >= 0) c_.kt.end(c_.stream, tok, degmax, kbytes);
     } else {
     // third work buffer: reuse x once it is no longer read (after step 2)
     const T *vk = x, *vkm1 = nullptr;
     T *bufs[3] = {b[0], b[1], x};
     int nb = 0;
+    const long tok = degmax > 0 && !capture_ ? c_.kt.begin(c_.stream, kname.c_str()) : -1;
+    double kbytes = 0.0;
     for (int d = 1; d <= degmax; d++) {
       int nact = 0;
       while (nact < g && degs[(size_t)nact] >= d) nact++;
@@ -485,12 +537,15 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
       else
         cheb_step_blk<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd,
                          mus_.p, d, nact, c_.stream);
-      c_.kstats.filter_bytes += (double)filter_launch_bytes(A, (int)elem, nact, d == 1);
+      const double lb = (double)filter_launch_bytes(A, (int)elem, nact, d == 1);
+      kbytes += lb;
+      c_.kstats.filter_bytes += lb;
       c_.kstats.filter_steps += nact;
       c_.kstats.filter_launches++;
       vkm1 = vk;
       vk = vkp1;
     }
+    if (tok >= 0) c_.kt.end(c_.stream, tok, degmax, kbytes);
     }
     if constexpr (std::is_same<T, float>::value) {
       if (b32_)
@@ -510,6 +565,35 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
     run((double *)nullptr);
 }
 
+std::vector<std::vector<int>> BatchSolver::filter_groups(const std::vector<int> &act) const {
+  // groups of at most gmax slices that share the filter's interval map
+  // (cc, dd: FindPol sets them from each slice's own [lmin, lmax],
+  // SRC/chebpol.c:386-389, and a block step applies one map to all its
+  // columns), each sorted by degree (descending)
+  std::vector<int> order(act);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const pevsl_Polparams *pa = S_[(size_t)a].pol, *pb = S_[(size_t)b].pol;
+    if (pa->cc != pb->cc) return pa->cc < pb->cc;
+    if (pa->dd != pb->dd) return pa->dd < pb->dd;
+    return pa->deg > pb->deg;
+  });
+  // group width: the tuned one, else 8 (FP32) / 4 (FP64), which pEVSL_Autotune
+  // picks on the 7-point stencil (wider row gathers cost more per column)
+  const int gmax = c_.block_width ? c_.block_width
+                                  : (c_.filter_prec == PEVSL_PREC_FP32 ? 8 : 4);
+  std::vector<std::vector<int>> out;
+  for (size_t i = 0; i < order.size();) {
+    const pevsl_Polparams *p0 = S_[(size_t)order[i]].pol;
+    size_t j = i + 1;
+    while (j < order.size() && j - i < (size_t)gmax && S_[(size_t)order[j]].pol->cc == p0->cc &&
+           S_[(size_t)order[j]].pol->dd == p0->dd)
+      j++;
+    out.emplace_back(order.begin() + (long)i, order.begin() + (long)j);
+    i = j;
+  }
+  return out;
+}
+
 void BatchSolver::filter_step(const std::vector<int> &act, int k) {
   for (int s : act) ensure_column(s, k + 1);
   const bool single = act.size() == 1 || !c_.A || c_.block_width == 1;
@@ -520,18 +604,7 @@ void BatchSolver::filter_step(const std::vector<int> &act, int k) {
     }
     return;
   }
-  // groups of at most 64 slices, each sorted by degree (descending)
-  std::vector<int> order(act);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    return S_[(size_t)a].pol->deg > S_[(size_t)b].pol->deg;
-  });
-  // group width: the tuned one, else 8 (FP32) / 4 (FP64), which pEVSL_Autotune
-  // picks on the 7-point stencil (wider row gathers cost more per column)
-  const int gmax = c_.block_width ? c_.block_width
-                                  : (c_.filter_prec == PEVSL_PREC_FP32 ? 8 : 4);
-  for (size_t i = 0; i < order.size(); i += (size_t)gmax) {
-    std::vector<int> grp(order.begin() + (long)i,
-                         order.begin() + (long)std::min(order.size(), i + (size_t)gmax));
+  for (const std::vector<int> &grp : filter_groups(act)) {
     if (grp.size() == 1) {
       filter_into(grp[0], nullptr, k, k + 1);
     } else {
@@ -553,10 +626,12 @@ void BatchSolver::lanczos_update(const std::vector<int> &act, int k) {
 // classical Gram-Schmidt with the DGKS test: pass 0 always, pass 1 only
 // for slices whose norm dropped below 1/sqrt(2) (decided on the device)
 void BatchSolver::cgs(const std::vector<int> &act, int k) {
-  lan_cgs_pass(L_, k, 0, c_.stream);
-  lan_cgs_pass(L_, k, 1, c_.stream);
   const double vecb = (double)vb_ * n_;
   const double pass = 2.0 * (k + 1) * vecb + 3 * vecb;
+  const long tok = c_.kt.begin(c_.stream, b32_ ? "cgs_pass<float>" : "cgs_pass<double>");
+  lan_cgs_pass(L_, k, 0, c_.stream);
+  c_.kt.end(c_.stream, tok, 1, act.size() * pass);
+  lan_cgs_pass(L_, k, 1, c_.stream);
   c_.kstats.reorth_bytes += act.size() * pass;
   c_.kstats.cgs_bytes += act.size() * pass;
   c_.kstats.cgs_passes += (long long)act.size();
@@ -675,6 +750,9 @@ void BatchSolver::ritz(int s, double tol) {
   const int csz = 1 << L_.chunk_shift;
   const double one = 1.0, zero = 0.0;
   const double acc_tol = accept_tol(s, tol);
+  // accepted vectors are copied straight into sl.vecs by asynchronous D2H
+  // copies: its storage must not move while they are in flight
+  if (c_.keep_vectors) sl.vecs.reserve(sl.vecs.size() + (size_t)nsel * n_);
   for (int cs = 0; cs < nsel && sl.rc == 0; cs += CH) {
     const int nb = std::min(CH, nsel - cs);
     // U = V(:, 0:kdim) * C(:, i0+cs : i0+cs+nb), accumulated over basis chunks
@@ -845,6 +923,7 @@ void BatchSolver::ritz_rr(int s, double tol) {
   PV_HIP(hipMemcpyAsync(rr.data(), red.p, sizeof(double) * nsel, hipMemcpyDeviceToHost, st));
   PV_HIP(hipStreamSynchronize(st));
   const double acc_tol = accept_tol(s, tol);
+  if (c_.keep_vectors) sl.vecs.reserve(sl.vecs.size() + (size_t)nsel * n_);
   for (int i = 0; i < nsel; i++) {
     const double t = tq[(size_t)i];
     if (t < sl.aa - DBL_EPSILON || t > sl.bb + DBL_EPSILON) continue;
@@ -865,11 +944,10 @@ void BatchSolver::ritz_rr(int s, double tol) {
   c_.kstats.t_ritz += wtime() - t0;
 }
 
-int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit,
-                     pevsl_Polparams *pols, FILE *fstats) {
-  n_ = p_->n;
+// Slice set-up shared by run() and bench(): filters, bounds, the batch
+// workspace and the pinned staging; -1 on an invalid filter centre.
+int BatchSolver::prepare(double *intvs, int *maxits, pevsl_Polparams *pols, double tall) {
   const hipStream_t st = c_.stream;
-  const double tall = wtime();
   int kmax = 0;
   for (int s = 0; s < ns_; s++) {
     Slice &sl = S_[(size_t)s];
@@ -925,6 +1003,16 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
   L_.part = part_.ensure(L_.part_cap);
   PV_HIP(hipHostMalloc(&pinned_, sizeof(double) * (2 * ns_ + 8)));
   PV_HIP(hipHostMalloc(&pinned_i_, sizeof(int) * (ns_ + 8)));
+  return 0;
+}
+
+int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit,
+                     pevsl_Polparams *pols, FILE *fstats) {
+  n_ = p_->n;
+  const hipStream_t st = c_.stream;
+  const double tall = wtime();
+  const double t_kern0 = c_.kt.seconds_matching("k_");
+  if (prepare(intvs, maxits, pols, tall) < 0) return -1;
 
   std::vector<int> act;
   for (int s = 0; s < ns_; s++)
@@ -1058,6 +1146,16 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
   }
   p_->stats->t_polAv += t_filter;
   p_->stats->t_reorth += t_reorth;
+  // products with A: every filter degree and every Ritz residual
+  // (pEVSL_MatvecA's count, INC/pevsl_int.h:83-89); filter applications:
+  // the start vector, one per step and one per breakdown (pEVSL_ChebAv)
+  for (int s = 0; s < ns_; s++) {
+    const Slice &sl = S_[(size_t)s];
+    if (c_.A) p_->stats->n_mvA += (size_t)sl.nmatvec;  // apply_A counts the callback's
+    if (sl.its > 0) p_->stats->n_polAv += (size_t)(1 + sl.its + sl.nbreak);
+  }
+  // time of the products: the fused filter kernels' HIP-event time
+  p_->stats->t_mvA += c_.kt.seconds_matching("k_") - t_kern0;
   c_.kstats.t_filter += t_filter;
   c_.kstats.t_reorth += t_reorth;
   c_.kstats.t_cgs += t_cgs;
@@ -1075,6 +1173,7 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
     in.nmatvec = sl.nmatvec;
     in.tr = sl.tr1;
     in.t_total = wtime() - tall;
+    in.tol_accept = accept_tol(s, tol);
   }
   PV_HIP(hipHostFree(pinned_));
   PV_HIP(hipHostFree(pinned_i_));
@@ -1084,6 +1183,124 @@ int BatchSolver::run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit
   for (int s = 0; s < ns_; s++)
     if (S_[(size_t)s].rc) rc = S_[(size_t)s].rc;
   return rc;
+}
+
+// Timed steps of a batched solve at a fixed basis size: every slice holds
+// k + 2 basis columns (zero, with a random unit column k as the filter's
+// input), and each step runs what one Lanczos iteration of the solve runs --
+// the filter groups, the three-term update and the Gram-Schmidt pass against
+// k + 1 columns -- followed by the solve's per-step synchronization.  The
+// instance's kernel statistics and per-kernel timers are reset after the
+// warm-up steps, so that they cover the timed steps only.  With graph_reps,
+// the first filter group's launch sequence (pack, degree steps, unpack) is
+// also captured in a HIP graph and replayed, against as many eager calls.
+int BatchSolver::bench(double *intvs, pevsl_Polparams *pols, int k, int warmup, int reps,
+                       int graph_reps, pevsl_StepBench *out) {
+  n_ = p_->n;
+  const hipStream_t st = c_.stream;
+  std::vector<int> maxits((size_t)ns_, k + 2);
+  if (prepare(intvs, maxits.data(), pols, wtime()) < 0) return -1;
+  std::vector<int> act;
+  for (int s = 0; s < ns_; s++)
+    if (!S_[(size_t)s].done) act.push_back(s);
+  if (act.empty()) return -2;
+  double *tmp = c_.cb_y.ensure((size_t)n_);
+  DBuf<double> pt;
+  pt.ensure((size_t)dot_part_blocks(n_) + 8);
+  for (int s : act) {
+    ensure_column(s, k + 1);
+    rand_fill((size_t)n_, tmp, 0x5eedULL + (uint64_t)s, 0, 0, st);
+    const int nb = dot_partials(n_, tmp, tmp, pt.p, st);
+    reduce_partials(pt.p, nb, 1, pt.p + nb, st);
+    PV_HIP(hipMemcpyAsync(pinned_, pt.p + nb, sizeof(double), hipMemcpyDeviceToHost, st));
+    PV_HIP(hipStreamSynchronize(st));
+    scale_vec(n_, tmp, nullptr, 1.0 / sqrt(pinned_[0]), st);
+    if (b32_)
+      copy_f64_to_f32((size_t)n_, tmp, colf(s, k), st);
+    else
+      PV_HIP(hipMemcpyAsync(cold(s, k), tmp, sizeof(double) * n_, hipMemcpyDeviceToDevice, st));
+  }
+  EvTimer ev_all, ev_f, ev_r, ev_c;
+  double t_all = 0, t_f = 0, t_r = 0, t_c = 0, wall = 0;
+  for (int r = 0; r < warmup + reps; r++) {
+    if (r == warmup) {
+      PV_HIP(hipStreamSynchronize(st));
+      memset(&c_.kstats, 0, sizeof(c_.kstats));
+      c_.kt.reset();
+    }
+    const double w0 = wtime();
+    ev_all.start(st);
+    ev_f.start(st);
+    filter_step(act, k);
+    ev_f.stop(st);
+    ev_r.start(st);
+    lanczos_update(act, k);
+    ev_c.start(st);
+    cgs(act, k);
+    ev_c.stop(st);
+    ev_r.stop(st);
+    ev_all.stop(st);
+    PV_HIP(hipStreamSynchronize(st));
+    if (r >= warmup) {
+      wall += wtime() - w0;
+      t_all += ev_all.seconds();
+      t_f += ev_f.seconds();
+      t_r += ev_r.seconds();
+      t_c += ev_c.seconds();
+    }
+  }
+  memset(out, 0, sizeof(*out));
+  out->nslices = (int)act.size();
+  out->k = k;
+  out->reps = reps;
+  if (reps > 0) {
+    out->sec_step = t_all / reps;
+    out->wall_step = wall / reps;
+    out->sec_filter = t_f / reps;
+    out->sec_reorth = t_r / reps;
+    out->sec_cgs = t_c / reps;
+    out->filter_bytes = c_.kstats.filter_bytes / reps;
+    out->cgs_bytes = c_.kstats.cgs_bytes / reps;
+    out->filter_launches = c_.kstats.filter_launches / reps;
+  }
+  const std::vector<std::vector<int>> groups = filter_groups(act);
+  if (graph_reps > 0 && act.size() > 1 && c_.A && c_.block_width != 1 && groups[0].size() > 1) {
+    const std::vector<int> &grp = groups[0];
+    const pevsl_KernelStats ks = c_.kstats;
+    filter_block(grp, k);  // stages the group's coefficients and pointers
+    PV_HIP(hipStreamSynchronize(st));
+    double w0 = wtime();
+    for (int r = 0; r < graph_reps; r++) filter_block(grp, k);
+    PV_HIP(hipStreamSynchronize(st));
+    out->eager_group_sec = (wtime() - w0) / graph_reps;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    capture_ = true;
+    PV_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    filter_block(grp, k);
+    PV_HIP(hipStreamEndCapture(st, &graph));
+    capture_ = false;
+    PV_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    PV_HIP(hipGraphLaunch(exec, st));  // first replay (uploads the graph)
+    PV_HIP(hipStreamSynchronize(st));
+    w0 = wtime();
+    for (int r = 0; r < graph_reps; r++) PV_HIP(hipGraphLaunch(exec, st));
+    PV_HIP(hipStreamSynchronize(st));
+    out->graph_group_sec = (wtime() - w0) / graph_reps;
+    size_t nn = 0;
+    PV_HIP(hipGraphGetNodes(graph, nullptr, &nn));
+    out->graph_nodes = (int)nn;
+    out->graph_group_width = (int)grp.size();
+    PV_HIP(hipGraphExecDestroy(exec));
+    PV_HIP(hipGraphDestroy(graph));
+    c_.kstats = ks;
+  }
+  for (int s : act) free_basis(s);
+  PV_HIP(hipHostFree(pinned_));
+  PV_HIP(hipHostFree(pinned_i_));
+  pinned_ = nullptr;
+  pinned_i_ = nullptr;
+  return 0;
 }
 
 void BatchSolver::outputs(int *nevOut, double **lamo, pevsl_Parvecs **Wo, double **reso,
@@ -1170,6 +1387,16 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs, int *max
   return rc;
 }
 
+int pEVSL_BenchBatchStep(pevsl_Data *pevsl, int nslices, double *intvs, pevsl_Polparams *pols,
+                         int k, int warmup, int reps, int graph_reps, pevsl_StepBench *out) {
+  Ctx &c = ctx_of(pevsl);
+  if (!c.A || pevsl->ifGenEv || nslices <= 0 || k < 1 || k + 2 > pevsl->N || warmup < 0 ||
+      reps < 0 || graph_reps < 0 || !out)
+    return -1;
+  BatchSolver bs(c, pevsl, nslices);
+  return bs.bench(intvs, pols, k, warmup, reps, graph_reps, out);
+}
+
 int pEVSL_ChebAv(pevsl_Data *pevsl, pevsl_Polparams *pol, pevsl_Parvec *v,
                  pevsl_Parvec *y, pevsl_Parvec *w) {
   (void)w;
@@ -1191,6 +1418,7 @@ int pEVSL_ChebAv(pevsl_Data *pevsl, pevsl_Polparams *pol, pevsl_Parvec *v,
   PV_HIP(hipStreamSynchronize(c.stream));
   pevsl->stats->t_polAv += wtime() - t0;
   pevsl->stats->n_polAv++;
+  if (c.A && !pevsl->ifGenEv) pevsl->stats->n_mvA += (size_t)pol->deg;
   return 0;
 }
 

@@ -302,6 +302,21 @@ class Instance:
         self.lib.pEVSL_GetKernelStats(self.p, C.byref(ks))
         return {f: getattr(ks, f) for f, _ in capi.KernelStats._fields_}
 
+    def kernel_times(self):
+        """Per-kernel HIP-event timings since the last stats reset:
+        {name: {"launches", "seconds", "bytes"}}."""
+        cnt = C.c_int(0)
+        self.lib.pEVSL_GetKernelTimes(self.p, 0, None, C.byref(cnt))
+        arr = (capi.KernelTime * max(1, cnt.value))()
+        self.lib.pEVSL_GetKernelTimes(self.p, cnt.value, arr, C.byref(cnt))
+        return {arr[i].name.decode(): {"launches": arr[i].launches, "seconds": arr[i].seconds,
+                                       "bytes": arr[i].bytes} for i in range(cnt.value)}
+
+    def stats(self):
+        """The instance's pevsl_Stat as a dict."""
+        st = self.p.contents.stats.contents
+        return {f: getattr(st, f) for f, _ in capi.Stat._fields_}
+
     def stats_reset(self):
         self.lib.pEVSL_StatsReset(self.p)
 

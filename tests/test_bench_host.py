@@ -1,9 +1,10 @@
-"""Host-side checks of bench.py that need no GPU: the multi-GPU refusal and
-the CPU-baseline leg (the oracle timed on a bounded sample and projected to
-a full slice)."""
+"""Host-side checks of bench.py that need no GPU: the multi-GPU refusal, the
+CPU-baseline sample on the oracle, and the FP64 projection model."""
 import os
 import subprocess
 import sys
+
+import numpy as np
 
 import bench
 
@@ -18,14 +19,32 @@ def test_more_than_one_gpu_is_refused():
     assert r.stdout.strip() == ""
 
 
-def test_cpu_baseline_projection():
-    """20^3 slice [0.6, 0.7]: the sample runs, reports the thread count and
-    projects T = its * t_step + t_cgs(512)/512 * its^2/2."""
-    its, found = 400, 20
-    cb = bench.cpu_baseline(20, 0.6, 0.7, 22, its, found)
-    assert cb["kind"] == "port" and cb["unit"] == "eigenpairs/s"
-    assert cb["cores"] >= 1
-    T = its * cb["t_step_s"] + cb["t_cgs512_s"] / 512.0 * its * its / 2.0
-    assert abs(cb["projected_slice_s"] - T) <= 1e-9 * T
-    assert abs(cb["value"] - found / T) <= 1e-9 * cb["value"]
-    assert "%d OpenMP threads" % cb["cores"] in cb["sample"]
+def test_cpu_sample_units():
+    """20^3 slice [0.6, 0.7]: the oracle sample reports a positive time per
+    degree step at the slice's FindPol degree, a Gram-Schmidt time per column
+    and the thread count."""
+    smp = bench.cpu_sample(20, 0.6, 0.7, 0, 22)
+    assert smp["deg"] > 1 and smp["t_degree_step"] > 0 and smp["t_cgs_per_col"] > 0
+    assert smp["threads"] >= 1 and smp["steps"] == 2
+
+
+def test_fp64_model_reproduces_its_inputs():
+    """The per-slice model T = cf deg (its+1) + cr its^2/2 + co its, fitted on
+    two slices, returns their own total when projected on them."""
+    meas = [{"deg": 150, "its": 2000, "t_filter": 150 * 2001 * 2.1e-5, "t_reorth": 2000 ** 2 * 6e-7,
+             "seconds": 150 * 2001 * 2.1e-5 + 2000 ** 2 * 6e-7 + 0.3},
+            {"deg": 450, "its": 3000, "t_filter": 450 * 3001 * 2.1e-5, "t_reorth": 3000 ** 2 * 6e-7,
+             "seconds": 450 * 3001 * 2.1e-5 + 3000 ** 2 * 6e-7 + 0.45}]
+    tot, coef = bench.fp64_model(meas, [(150, 2000), (450, 3000)])
+    assert abs(coef["cf_s_per_degree_step"] - 2.1e-5) < 1e-12
+    assert abs(coef["cr_s_per_step2"] - 1.2e-6) < 1e-15
+    assert abs(tot - sum(m["seconds"] for m in meas)) < 1e-9 * tot
+    assert np.isfinite(tot)
+
+
+def test_roofline_picks_the_kernel_with_most_time():
+    kt = {"k_a<float,8>": {"launches": 10, "seconds": 1e-3, "bytes": 1.875e9},
+          "k_b<float,4>": {"launches": 10, "seconds": 5e-4, "bytes": 1e9}}
+    r = bench.roofline_of(kt, {"k_a<float,8>": 2e8})
+    assert r["kernel"] == "k_a<float,8>" and r["traffic"] == 2e8
+    assert abs(r["achieved"] - 1875.0) < 1e-6 and abs(r["frac"] - 1875.0 / 8000.0) < 1e-4
