@@ -270,6 +270,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe", choices=["lap", "irregular"], default=None)
+    ap.add_argument("--graph-reps", type=int, default=20,
+                    help="replays of the captured filter group in the hipGraph-vs-eager "
+                         "comparison (0: no comparison)")
+    ap.add_argument("--progress", action="store_true",
+                    help="the solver's convergence-test lines (ChebLanNr fstats) on stderr")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this library runs on one GPU only (multi-GPU sharding is "
@@ -304,6 +309,7 @@ def main():
     left = lambda: args.budget - (time.time() - T_START)  # noqa: E731
 
     lib = capi.load()
+    fst = driver.c_stderr() if args.progress else None
     name = C.create_string_buffer(128)
     hbm = C.c_size_t(0)
     lib.pEVSL_DeviceInfo(name, 128, C.byref(hbm))
@@ -352,7 +358,8 @@ def main():
         log("slicing: imbalance %.3f, degrees %s" % (imb, degs))
         its = None
         if not args.no_job:
-            rc, outs = driver.cheblan_batch(inst, intvs, [mlan] * ns, tol, vinit, pols)
+            rc, outs = driver.cheblan_batch(inst, intvs, [mlan] * ns, tol, vinit, pols,
+                                            fstats=fst)
             assert rc == 0, rc
             tjob = time.time() - tj
             ks = inst.kernel_stats()
@@ -392,7 +399,7 @@ def main():
         ivarr = np.ascontiguousarray(np.asarray(intvs, dtype=np.float64).reshape(ns, 4))
         parr = (capi.Polparams * ns)(*pols)
         rc = lib.pEVSL_BenchBatchStep(inst.p, ns, ivarr.ctypes.data_as(capi.c_double_p), parr,
-                                      k, args.warmup, args.steps, 20, C.byref(sb))
+                                      k, args.warmup, args.steps, args.graph_reps, C.byref(sb))
         assert rc == 0, rc
         kt = inst.kernel_times()
         out["ms_per_step"] = round(sb.sec_step * 1e3, 3)
@@ -453,7 +460,8 @@ def main():
             for s in sub:
                 lib.pEVSL_StatsReset(inst64.p)
                 t1 = time.time()
-                rc, o = driver.cheblan_batch(inst64, [intvs[s]], [mlan], tol, vinit, [pols[s]])
+                rc, o = driver.cheblan_batch(inst64, [intvs[s]], [mlan], tol, vinit, [pols[s]],
+                                             fstats=fst)
                 assert rc == 0, rc
                 dt = time.time() - t1
                 ks = inst64.kernel_stats()

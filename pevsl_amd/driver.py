@@ -382,10 +382,18 @@ def c_stdout():
     return C.c_void_p.in_dll(libc, "stdout")
 
 
+def c_stderr():
+    """libc's stderr FILE* (unbuffered; for the fstats argument)."""
+    libc = C.CDLL(None)
+    return C.c_void_p.in_dll(libc, "stderr")
+
+
 def cheblan_batch(inst, intvs, maxits, tol, vinit, pols, keep_vectors=False,
-                  verbose=False):
+                  verbose=False, fstats=None):
     """pEVSL_ChebLanNrBatch over len(pols) slices.  Returns a list of dicts
-    with lam (unsorted, reference order), res, vecs (or None) and info."""
+    with lam (unsorted, reference order), res, vecs (or None) and info.
+    The solver's per-test lines go to stdout with verbose, or to the FILE*
+    `fstats`."""
     lib = inst.lib
     ns = len(pols)
     iv = np.ascontiguousarray(np.asarray(intvs, dtype=np.float64).reshape(ns, 4))
@@ -403,7 +411,8 @@ def cheblan_batch(inst, intvs, maxits, tol, vinit, pols, keep_vectors=False,
     rc = lib.pEVSL_ChebLanNrBatch(inst.p, ns, iv.ctypes.data_as(capi.c_double_p),
                                   mi.ctypes.data_as(capi.c_int_p), tol, C.byref(pv),
                                   parr, nev.ctypes.data_as(capi.c_int_p), lamo, Wo,
-                                  reso, c_stdout() if verbose else None)
+                                  reso, fstats if fstats is not None
+                                  else (c_stdout() if verbose else None))
     libc = C.CDLL(None)
     libc.free.argtypes = [C.c_void_p]
     out = []
