@@ -183,6 +183,21 @@ int pEVSL_GetKernelTimes(pevsl_Data *pevsl, int maxn, pevsl_KernelTime *out, int
  * until done. */
 int pEVSL_DeviceMatvec(pevsl_Data *pevsl, const double *d_x, double *d_y);
 
+/* y = p((A - cc)/dd) v as pEVSL_ChebAv, on vectors already in HBM (d_v,
+ * d_y: n doubles each), allocating nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The filter's work
+ * vectors live in a library-owned device array of 3 * 2^21 doubles (48 MiB)
+ * per GPU, held until the process ends, so n is limited to
+ * pEVSL_ChebAvNoAllocMaxRows() = 2^21 rows; larger n returns -5 with a
+ * message.  Needs a device matrix (pEVSL_SetAParcsr) and the standard
+ * problem, else PEVSL_NOT_IMPLEMENT.  Same filter kernels and result as
+ * pEVSL_ChebAv; blocks until done.  All calls on one GPU share the array:
+ * calls that overlap on one GPU race. */
+int pEVSL_ChebAvNoAlloc(pevsl_Data *pevsl, pevsl_Polparams *pol, const double *d_v,
+                        double *d_y);
+size_t pEVSL_ChebAvNoAllocMaxRows(void);
+
 /* Device name and HBM size of the current device; 0 on success. */
 int pEVSL_DeviceInfo(char *name, int len, size_t *hbm_bytes);
 

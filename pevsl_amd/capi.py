@@ -309,6 +309,8 @@ SIGNATURES = {
     "pEVSL_GetKernelTimes": (C.c_int, [DataP, C.c_int, C.POINTER(KernelTime),
                                        C.POINTER(C.c_int)]),
     "pEVSL_DeviceMatvec": (C.c_int, [DataP, C.c_void_p, C.c_void_p]),
+    "pEVSL_ChebAvNoAlloc": (C.c_int, [DataP, C.POINTER(Polparams), C.c_void_p, C.c_void_p]),
+    "pEVSL_ChebAvNoAllocMaxRows": (C.c_size_t, []),
     "pEVSL_DeviceInfo": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_size_t)]),
     "pEVSL_SymmTridEigvals": (C.c_int, [C.c_int, c_double_p, c_double_p, C.c_double,
                                         c_double_p, c_int_p, c_double_p]),

@@ -183,9 +183,11 @@ void apply_A(Ctx &c, const double *dx, double *dy);
 void rand_vectors(Ctx &c, int n, int nvec, double *d_out, size_t ld);
 
 // Chebyshev filter y = p((A-cc)/dd) v on the device for one FP64 column
-// (v, y in HBM).  Work vectors come from the context.
+// (v, y in HBM).  Work vectors come from the context, or, when `work` is
+// given, from that device buffer of 3n doubles (device matrix only); the call
+// then allocates nothing and records no per-kernel timers.
 void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
-                     double *dy);
+                     double *dy, double *work = nullptr);
 // The same for FP32 columns (FP32 filter arithmetic; device matrix only).
 void cheb_filter_col32(Ctx &c, const pevsl_Polparams *pol, const float *dv, float *dy);
 

@@ -106,8 +106,9 @@ std::string filter_kernel_name(bool f32, int kind, int B, bool stream, int lanes
 }  // namespace
 
 // FP32 filter y = p((A - cc)/dd) v on FP32 columns; w holds 3n floats.
+// `timed`: record the run in the instance's per-kernel HIP-event timers.
 static void cheb_filter_f32(Ctx &c, const pevsl_Polparams *pol, const float *x32, float *y32,
-                            float *w) {
+                            float *w, bool timed = true) {
   const DevCsr &A = *c.A;
   const int n = A.n, m = pol->deg;
   const double *mu = pol->mu;
@@ -118,8 +119,9 @@ static void cheb_filter_f32(Ctx &c, const pevsl_Polparams *pol, const float *x32
   float *b[3] = {w, w + (size_t)n, w + 2 * (size_t)n};
   const float *vk = x32, *vkm1 = nullptr;
   int nb = 0;
-  const long tok = m > 0 ? c.kt.begin(s, filter_kernel_name(true, 0, 0, stream, lanes).c_str())
-                         : -1;
+  const long tok = m > 0 && timed
+                       ? c.kt.begin(s, filter_kernel_name(true, 0, 0, stream, lanes).c_str())
+                       : -1;
   double bytes = 0.0;
   for (int k = 1; k <= m; k++) {
     float *vkp1 = b[nb];
@@ -142,15 +144,18 @@ static void cheb_filter_f32(Ctx &c, const pevsl_Polparams *pol, const float *x32
 
 // y = p((A - cc)/dd) v for one FP64 column in HBM (SRC/chebpol.c:509-574).
 void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
-                     double *dy) {
+                     double *dy, double *work) {
   const int n = c.A ? c.A->n : (int)c.h_x.size();
   const int m = pol->deg;
   const double *mu = pol->mu;
   const double cc = pol->cc, dd = pol->dd;
   const hipStream_t s = c.stream;
+  if (work && !c.A) fatal("cheb_filter_col: caller work vectors need a device matrix",
+                          __FILE__, __LINE__);
   if (m == 0 || !c.A) {
-    // operator given as a host callback (or trivial degree): unfused steps
-    double *w = c.scratch64.ensure((size_t)4 * n);
+    // operator given as a host callback (or trivial degree): unfused steps;
+    // with a device matrix this branch only runs at degree 0 (one vector)
+    double *w = work ? work : c.scratch64.ensure((size_t)4 * n);
     double *vk = w, *vkp1 = w + n, *vkm1 = w + 2 * (size_t)n;
     PV_HIP(hipMemcpyAsync(vk, dv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     PV_HIP(hipMemcpyAsync(dy, vk, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -174,11 +179,12 @@ void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
   // vector-CSR lane count on the 7-point stencil)
   const bool stream = c.use_stream != 0;
   if (c.filter_prec == PEVSL_PREC_FP64) {
-    double *w = c.scratch64.ensure((size_t)3 * n);
+    double *w = work ? work : c.scratch64.ensure((size_t)3 * n);
     double *b[3] = {w, w + n, w + 2 * (size_t)n};
     const double *vk = dv, *vkm1 = nullptr;
     int nb = 0;
-    const long tok = c.kt.begin(s, filter_kernel_name(false, 0, 0, stream, lanes).c_str());
+    const long tok =
+        work ? -1 : c.kt.begin(s, filter_kernel_name(false, 0, 0, stream, lanes).c_str());
     const double bytes0 = c.kstats.filter_bytes;
     for (int k = 1; k <= m; k++) {
       double *vkp1 = b[nb];
@@ -194,12 +200,14 @@ void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
       vk = vkp1;
       c.kstats.filter_bytes += (double)filter_launch_bytes(A, 8, 1, k == 1);
     }
-    c.kt.end(s, tok, m, c.kstats.filter_bytes - bytes0);
+    if (tok >= 0) c.kt.end(s, tok, m, c.kstats.filter_bytes - bytes0);
   } else {
-    float *w = reinterpret_cast<float *>(c.scratch64.ensure((size_t)5 * n / 2 + 8));
+    // x32, y32 and three work columns: 5n floats (within 3n doubles)
+    float *w = reinterpret_cast<float *>(work ? work
+                                              : c.scratch64.ensure((size_t)5 * n / 2 + 8));
     float *x32 = w, *y32 = w + n;
     copy_f64_to_f32((size_t)n, dv, x32, s);
-    cheb_filter_f32(c, pol, x32, y32, w + 2 * (size_t)n);
+    cheb_filter_f32(c, pol, x32, y32, w + 2 * (size_t)n, work == nullptr);
     copy_f32_to_f64((size_t)n, y32, dy, s);
     return;
   }

@@ -21,7 +21,7 @@ def declared_functions():
     for h in ("pevsl.h", "pevsl_amd.h", "pevsl_itsol.h", "pevsl_f90.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-        for m in re.finditer(r"^\s*(?:int|void|double)\s+\*?\s*(\w+)\s*\(", txt, re.M):
+        for m in re.finditer(r"^\s*(?:int|void|double|size_t)\s+\*?\s*(\w+)\s*\(", txt, re.M):
             names.add(m.group(1))
     return names
 
