@@ -131,7 +131,10 @@ int pEVSL_SetSpmvTuning(pevsl_Data *pevsl, int lanes_per_row, int block_width);
  * accumulator (5 vector passes per degree), 2 = the same kernel in Clenshaw
  * form (4 passes), 3 / 4 = the nnz-balanced forms of 1 / 2 (products formed
  * in parallel over nonzeros and staged in LDS, long rows reduced by a whole
- * workgroup: for uneven row lengths), 0 = one lane per (row, column).
+ * workgroup: for uneven row lengths), 0 = one lane per (row, column),
+ * 5 = the Clenshaw form with one thread per (row, 16-byte piece of columns):
+ * up to 64 FP32 columns share one pass over the matrix, each piece active
+ * from its own top degree (FP32 filter only; 1 on FP64).
  * Default (-1): 2 for an FP32 filter, 1 for FP64, or 4 / 3 when the
  * matrix has a row longer than 64 entries. */
 int pEVSL_SetBlockKernel(pevsl_Data *pevsl, int rowkernel);
@@ -154,6 +157,19 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs,
                          int *maxits, double tol, pevsl_Parvec *vinit,
                          pevsl_Polparams *pols, int *nevOut, double **lamo,
                          pevsl_Parvecs **Wo, double **reso, FILE *fstats);
+
+/* Slices per wave of pEVSL_ChebLanNrBatch (0, the default: all slices in
+ * one batch).  With wave > 0 a call with more slices solves them in
+ * ceil(nslices / wave) batches of balanced size, in order of filter degree,
+ * one after the other: peak device memory is then that of the largest wave
+ * (each slice's Lanczos basis grows to n x its step count, in 8-byte FP64
+ * or 4-byte FP32 elements).  Outputs are per slice as for one batch. */
+int pEVSL_SetBatchWave(pevsl_Data *pevsl, int wave);
+
+/* Return code of pEVSL_ChebLanNrBatch / pEVSL_ChebLanNr when a Lanczos
+ * basis does not fit in device memory; the call releases what it held and
+ * returns no eigenpairs (nevOut 0, NULL outputs). */
+#define PEVSL_ERR_DEVICE_OOM (-7)
 
 /* y = p((A - cc)/dd) v for host vectors v, y (w unused: the work vectors
  * live on the device).  Same arguments as the reference's internal
@@ -225,6 +241,7 @@ typedef struct {
   double bytes_clen[6];   /* its algorithmic bytes per step */
   double t_blockbal[6];   /* nnz-balanced three-term kernel, widths 2..64 */
   double t_blockbalclen[6];  /* nnz-balanced Clenshaw kernel (FP32 only) */
+  double t_blockseg[6];   /* segmented Clenshaw kernel, widths 4..64 (FP32 only) */
 } pevsl_TuneResult;
 int pEVSL_Autotune(pevsl_Data *pevsl, int reps, pevsl_TuneResult *res);
 

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libpevsl_amd.so")
 
 MPI_COMM_WORLD = 0
 PEVSL_NOT_IMPLEMENT = 11
+PEVSL_ERR_DEVICE_OOM = -7
 PEVSL_PREC_FP64 = 0
 PEVSL_PREC_FP32 = 1
 PEVSL_RAND_LIBC = 0
@@ -100,7 +101,7 @@ class TuneResult(C.Structure):
                 ("t_stream", C.c_double), ("t_blockrow", C.c_double * 6),
                 ("block_rowkernel", C.c_int), ("t_blockclen", C.c_double * 6),
                 ("bytes_clen", C.c_double * 6), ("t_blockbal", C.c_double * 6),
-                ("t_blockbalclen", C.c_double * 6)]
+                ("t_blockbalclen", C.c_double * 6), ("t_blockseg", C.c_double * 6)]
 
 
 class StepBench(C.Structure):
@@ -310,6 +311,7 @@ SIGNATURES = {
                                        C.POINTER(C.c_int)]),
     "pEVSL_DeviceMatvec": (C.c_int, [DataP, C.c_void_p, C.c_void_p]),
     "pEVSL_ChebAvNoAlloc": (C.c_int, [DataP, C.POINTER(Polparams), C.c_void_p, C.c_void_p]),
+    "pEVSL_SetBatchWave": (C.c_int, [DataP, C.c_int]),
     "pEVSL_ChebAvNoAllocMaxRows": (C.c_size_t, []),
     "pEVSL_DeviceInfo": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_size_t)]),
     "pEVSL_SymmTridEigvals": (C.c_int, [C.c_int, c_double_p, c_double_p, C.c_double,

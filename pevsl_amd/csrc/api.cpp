@@ -380,8 +380,14 @@ int pEVSL_SetSpmvTuning(pevsl_Data *p, int lanes, int bw) {
   return 0;
 }
 
+int pEVSL_SetBatchWave(pevsl_Data *p, int wave) {
+  if (wave < 0) return -1;
+  ctx_of(p).batch_wave = wave;
+  return 0;
+}
+
 int pEVSL_SetBlockKernel(pevsl_Data *p, int rowkernel) {
-  if (rowkernel < -1 || rowkernel > 4) return -1;
+  if (rowkernel < -1 || rowkernel > 5) return -1;
   ctx_of(p).blk_rowkernel = rowkernel;
   return 0;
 }

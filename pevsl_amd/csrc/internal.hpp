@@ -129,6 +129,7 @@ struct Ctx {
   int blk_rowkernel = -1;          // block SpMM: 1 row-per-thread, 0 lane per (row, col), -1 auto
   int keep_vectors = 1;
   int chunk_cols = 512;            // Lanczos basis allocation granularity
+  int batch_wave = 0;              // ChebLanNrBatch: slices per wave (0: all at once)
   std::vector<pevsl_LanInfo> info; // per slice of the last (batched) solve
   pevsl_KernelStats kstats{};      // accumulated device timings
   KernTimers kt;                   // per-kernel HIP-event timings

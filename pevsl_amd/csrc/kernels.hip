@@ -15,6 +15,10 @@
 //     (row-major n x B), a/ja staged in LDS, one thread per row gathering
 //     whole B-wide rows with 16-byte loads; three-term form with an
 //     accumulator, or the Clenshaw form (four vector passes instead of five).
+//   * k_clen_seg : the Clenshaw step with one thread per (row, 16-byte piece
+//     of columns), up to 64 FP32 columns in one pass over the matrix, each
+//     piece active from its own top degree (uneven degrees cost their own
+//     steps only, at 4-column granularity).
 //   * k_blk_bal (epilogues EpiCheb / EpiClen) and k_kpm_bal : the same steps
 //     for matrices with uneven rows, parallel over (nonzero, 16-byte piece)
 //     with the products staged in LDS and summed per row in column order.
@@ -50,6 +54,8 @@ namespace pevsl {
 namespace {
 
 constexpr int kThreads = 256;
+// MI355X Infinity Cache (memory-side last-level cache)
+constexpr double kInfinityCacheBytes = 256.0 * 1024 * 1024;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -321,6 +327,35 @@ __device__ __forceinline__ void store_row(T *__restrict__ p, const T (&x)[B]) {
   }
 }
 
+// 16-byte pieces that are read or written once per launch (matrix, v, b2,
+// out): nontemporal, so that they do not evict the gathered rows of b1 from
+// L2 between their reuses (neighbours nx*ny rows apart on a 3-D grid)
+template <typename T, int W>
+__device__ __forceinline__ void load_row_nt(const T *__restrict__ p, T (&x)[W]) {
+  if constexpr ((W * sizeof(T)) % 16 == 0) {
+#pragma unroll
+    for (int i = 0; i < (int)(W * sizeof(T) / 16); i++) {
+      const fvec4 v = __builtin_nontemporal_load(reinterpret_cast<const fvec4 *>(p) + i);
+      reinterpret_cast<fvec4 *>(x)[i] = v;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < W; c++) x[c] = __builtin_nontemporal_load(p + c);
+  }
+}
+template <typename T, int W>
+__device__ __forceinline__ void store_row_nt(T *__restrict__ p, const T (&x)[W]) {
+  if constexpr ((W * sizeof(T)) % 16 == 0) {
+#pragma unroll
+    for (int i = 0; i < (int)(W * sizeof(T) / 16); i++)
+      __builtin_nontemporal_store(reinterpret_cast<const fvec4 *>(x)[i],
+                                  reinterpret_cast<fvec4 *>(p) + i);
+  } else {
+#pragma unroll
+    for (int c = 0; c < W; c++) __builtin_nontemporal_store(x[c], p + c);
+  }
+}
+
 template <typename T, int B>
 __global__ void __launch_bounds__(kThreads)
 k_cheb_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
@@ -445,6 +480,103 @@ k_clen_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
     w[c] = z + (T)(-1) * b2r[c];
   }
   store_row<T, B>(out + (size_t)r * B, w);
+}
+
+// One Clenshaw step on the active pieces of an interleaved block of Bp
+// columns (row stride Bp), one thread per (row, 16-byte piece of SEG
+// columns): the columns of a wide batch share one pass over the matrix while
+// each piece carries its own degree range.  Piece s is active while
+// d <= tops.top[s] (its highest degree); at d == top it writes the top term
+// mu_d v (no product), at d == top - 1 it reads b2 as zero, below that
+//   out = mu_d v + t (A b1 - cc b1) - b2
+// exactly as k_clen_blkrow computes each column (same operations and order,
+// hence the same bits).  Inactive pieces are neither read nor written: the
+// host launches the active prefix (pieces sorted by top degree, descending)
+// over a row partition with at most 256/Q rows and 2048/Q nonzeros per
+// block, Q >= nseg, so that every thread owns at most one (row, piece).
+// NT: the matrix, v, b2 and out pass through as nontemporal accesses, which
+// keeps L2 for the gathered rows of b1 when a launch's data exceeds the
+// Infinity Cache (measured on the 100^3 Laplacian, 32 columns: 149.6 ->
+// 132.1 us; at 8 columns, whose 187 MB stay cached from launch to launch,
+// the hint costs 30%: 32.6 -> 42.5 us).
+template <typename T, bool NT>
+__global__ void __launch_bounds__(kThreads)
+k_clen_seg(const int *__restrict__ rowblk, const int *__restrict__ ia,
+           const int *__restrict__ ja, const T *__restrict__ a,
+           const T *__restrict__ b1, const T *__restrict__ b2,
+           const T *__restrict__ v, T *__restrict__ out, T cc, T t,
+           const double *__restrict__ mus, int d, int Bp, int nseg, int maxnz,
+           SegTops tops) {
+#pragma clang fp contract(off)
+  constexpr int SEG = 16 / (int)sizeof(T);
+  __shared__ T as[kStreamNnz];
+  __shared__ int js[kStreamNnz];
+  const int rb = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = rowblk[rb], r1 = rowblk[rb + 1];
+  const int j0 = ia[r0], nz = ia[r1] - j0;
+  // pieces that need the product this step (top > d: piece 0 has the
+  // highest top)
+  const bool gather = tops.top[0] > d;
+  const bool staged = gather && nz <= maxnz;
+  if (staged) {
+    for (int q = threadIdx.x; q < nz; q += kThreads) {
+      as[q] = NT ? __builtin_nontemporal_load(a + j0 + q) : a[j0 + q];
+      js[q] = NT ? __builtin_nontemporal_load(ja + j0 + q) : ja[j0 + q];
+    }
+    __syncthreads();
+  }
+  const int lr = (int)threadIdx.x / nseg;
+  const int sg = (int)threadIdx.x - lr * nseg;
+  const int r = r0 + lr;
+  if (r >= r1) return;
+  const int top = tops.top[sg];
+  const int c0 = sg * SEG;
+  const size_t off = (size_t)r * Bp + c0;
+  const double *mu = mus + (size_t)d * Bp + c0;
+  T vv[SEG], w[SEG];
+  if (NT)
+    load_row_nt<T, SEG>(v + off, vv);
+  else
+    load_row<T, SEG>(v + off, vv);
+  if (d == top) {
+#pragma unroll
+    for (int c = 0; c < SEG; c++) w[c] = (T)mu[c] * vv[c];
+  } else {
+    T acc[SEG];
+#pragma unroll
+    for (int c = 0; c < SEG; c++) acc[c] = 0;
+    const int qs = ia[r] - j0, qe = ia[r + 1] - j0;
+    for (int q = qs; q < qe; q++) {
+      const T av = staged ? as[q] : a[j0 + q];
+      const int col = staged ? js[q] : ja[j0 + q];
+      T x[SEG];
+      load_row<T, SEG>(b1 + (size_t)col * Bp + c0, x);
+#pragma unroll
+      for (int c = 0; c < SEG; c++) acc[c] += av * x[c];
+    }
+    T bb[SEG], b2r[SEG];
+    load_row<T, SEG>(b1 + off, bb);
+    if (b2 && d < top - 1) {
+      if (NT)
+        load_row_nt<T, SEG>(b2 + off, b2r);
+      else
+        load_row<T, SEG>(b2 + off, b2r);
+    } else {
+#pragma unroll
+      for (int c = 0; c < SEG; c++) b2r[c] = 0;
+    }
+#pragma unroll
+    for (int c = 0; c < SEG; c++) {
+      T z = acc[c] + (-cc) * bb[c];
+      z = z * t;
+      z = z + (T)mu[c] * vv[c];
+      w[c] = z + (T)(-1) * b2r[c];
+    }
+  }
+  if (NT)
+    store_row_nt<T, SEG>(out + off, w);
+  else
+    store_row<T, SEG>(out + off, w);
 }
 
 // out[r*B + c] = mus[k*B + c] * v[r*B + c]  (top Clenshaw term)
@@ -1685,6 +1817,33 @@ template void clen_step_blkrow<float>(const DevCsr &, int, const float *, const 
 template void clen_step_blkrow<double>(const DevCsr &, int, const double *, const double *,
                                        const double *, double *, double, double,
                                        const double *, int, hipStream_t);
+
+template <typename T>
+void clen_step_seg(const DevCsr &A, int Bp, const T *b1, const T *b2, const T *v, T *out,
+                   double cc, double t, const double *mus, int d, int nseg, const SegTops &tops,
+                   hipStream_t s) {
+  if (A.n == 0 || nseg <= 0) return;
+  int l = 0;
+  while ((1 << l) < nseg) l++;
+  const int *rb = l == 0 ? A.rowblk : A.rowblk_q[l];
+  const int nb = l == 0 ? A.nrowblk : A.nrowblk_q[l];
+  if (!rb || nb == 0) return;
+  const T *a = std::is_same<T, double>::value ? (const T *)A.a64 : (const T *)A.a32;
+  // data of one full-width launch: the matrix and four Bp-wide vector passes
+  const double bytes = (double)A.nnz * (sizeof(T) + 4) + 4.0 * (double)Bp * A.n * sizeof(T);
+  if (bytes > kInfinityCacheBytes)
+    hipLaunchKernelGGL((k_clen_seg<T, true>), dim3(nb), dim3(kThreads), 0, s, rb, A.ia, A.ja,
+                       a, b1, b2, v, out, (T)cc, (T)t, mus, d, Bp, nseg, kStreamNnz >> l, tops);
+  else
+    hipLaunchKernelGGL((k_clen_seg<T, false>), dim3(nb), dim3(kThreads), 0, s, rb, A.ia, A.ja,
+                       a, b1, b2, v, out, (T)cc, (T)t, mus, d, Bp, nseg, kStreamNnz >> l, tops);
+}
+template void clen_step_seg<float>(const DevCsr &, int, const float *, const float *,
+                                   const float *, float *, double, double, const double *, int,
+                                   int, const SegTops &, hipStream_t);
+template void clen_step_seg<double>(const DevCsr &, int, const double *, const double *,
+                                    const double *, double *, double, double, const double *,
+                                    int, int, const SegTops &, hipStream_t);
 
 template void cheb_step_blkrow<double>(const DevCsr &, int, const double *,
                                        const double *, double *, double *, double,

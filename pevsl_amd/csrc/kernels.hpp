@@ -83,6 +83,22 @@ template <typename T>
 void clen_step_blkrow(const DevCsr &A, int B, const T *b1, const T *b2, const T *v, T *out,
                       double cc, double t, const double *mus, int k, hipStream_t s);
 
+// Clenshaw step on the active 16-byte pieces (SEG = 16/sizeof(T) columns
+// each) of an interleaved block of Bp columns (Bp a multiple of SEG, at most
+// kMaxSegs pieces), one thread per (row, piece): pieces 0..nseg-1 are active
+// (sorted by top degree, descending); piece s writes the top term mu_d v at
+// d == tops.top[s], reads b2 as zero at d == top - 1, and otherwise computes
+// out = mu_d v + t (A b1 - cc b1) - b2 with mus[d*Bp + c], as
+// clen_step_blkrow does per column.  Pieces >= nseg are not touched.
+constexpr int kMaxSegs = 16;
+struct SegTops {
+  int top[kMaxSegs];
+};
+template <typename T>
+void clen_step_seg(const DevCsr &A, int Bp, const T *b1, const T *b2, const T *v, T *out,
+                   double cc, double t, const double *mus, int d, int nseg, const SegTops &tops,
+                   hipStream_t s);
+
 // nnz-balanced forms of cheb_step_blkrow and clen_step_blkrow (same
 // arguments and results; products formed in parallel over nonzeros and
 // staged in LDS, long rows reduced by a whole workgroup); B <= 32.

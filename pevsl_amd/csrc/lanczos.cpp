@@ -76,6 +76,18 @@ size_t clen_launch_bytes(const DevCsr &A, int elem, int ncol, bool has_b2) {
                   (has_b2 ? 4.0 : 3.0) * vec * ncol);
 }
 
+// segmented Clenshaw step: the matrix once if any piece forms the product,
+// then per active piece of `seg` columns two (top term: v, out), three (b1,
+// v, out) or four vector passes
+double clen_seg_launch_bytes(const DevCsr &A, int elem, int seg, int d, int nseg,
+                             const SegTops &tops) {
+  const double piece = (double)A.n * elem * seg;
+  double b = tops.top[0] > d ? (double)A.nnz * (elem + 4) + 4.0 * (A.n + 1) : 0.0;
+  for (int s = 0; s < nseg; s++)
+    b += (d == tops.top[s] ? 2.0 : d == tops.top[s] - 1 ? 3.0 : 4.0) * piece;
+  return b;
+}
+
 size_t filter_launch_bytes(const DevCsr &A, int elem, int ncol, bool first) {
   const double vec = (double)A.n * elem;
   return (size_t)((double)A.nnz * (elem + 4) + 4.0 * (A.n + 1) +
@@ -91,6 +103,8 @@ std::string filter_kernel_name(bool f32, int kind, int B, bool stream, int lanes
       snprintf(nm, sizeof nm, "k_cheb_stream<%s>", t);
     else
       snprintf(nm, sizeof nm, "k_cheb_vec<%s,%d>", t, lanes);
+  } else if (kind == 5) {
+    snprintf(nm, sizeof nm, "k_clen_seg<%s,%d>", t, B);
   } else if (kind >= 3) {
     snprintf(nm, sizeof nm, "k_blk_bal<%s,%d,%s>", t, B, kind == 4 ? "EpiClen" : "EpiCheb");
   } else if (kind == 2) {
@@ -253,9 +267,25 @@ void gram_tn(rocblas_handle h, hipStream_t st, int n, int m1, const double *A, i
   sum_parts(wk, parts, blk, C, st);
 }
 
+// Raised when a basis chunk cannot be allocated: the batch does not fit in
+// device memory (pEVSL_ChebLanNrBatch returns PEVSL_ERR_DEVICE_OOM).
+struct DeviceOom {
+  size_t bytes;
+};
+
 class BatchSolver {
  public:
   BatchSolver(Ctx &c, pevsl_Data *p, int ns) : c_(c), p_(p), ns_(ns), S_(ns) {}
+  // releases the bases and staging buffers a run left (an interrupted run)
+  ~BatchSolver() {
+    (void)hipStreamSynchronize(c_.stream);
+    for (Slice &sl : S_)
+      for (void *q : sl.chunks) (void)hipFree(q);
+    if (pinned_) (void)hipHostFree(pinned_);
+    if (pinned_i_) (void)hipHostFree(pinned_i_);
+  }
+  BatchSolver(const BatchSolver &) = delete;
+  BatchSolver &operator=(const BatchSolver &) = delete;
 
   int run(double *intvs, int *maxits, double tol, pevsl_Parvec *vinit,
           pevsl_Polparams *pols, FILE *fstats);
@@ -301,6 +331,18 @@ class BatchSolver {
 
   // the filter groups of one step: at most the tuned width, one interval map
   std::vector<std::vector<int>> filter_groups(const std::vector<int> &act) const;
+  // block SpMM kernel of the batched filter: 0 lane per (row, column), 1
+  // row per thread, 2 its Clenshaw form, 3 / 4 the nnz-balanced forms of 1 /
+  // 2, 5 the segmented Clenshaw form (FP32).  Default: 2 on the FP32 path,
+  // the reference's three-term recurrence (1) on FP64, the balanced forms
+  // when some row is long.
+  int block_kind() const {
+    const bool f32 = c_.filter_prec == PEVSL_PREC_FP32;
+    int kind = c_.blk_rowkernel;
+    if (kind < 0) kind = (f32 ? 2 : 1) + (c_.A->max_row > kUnevenRow ? 2 : 0);
+    if (kind == 5 && !f32) kind = 1;  // Clenshaw forms: FP32 only
+    return kind;
+  }
 
   Ctx &c_;
   pevsl_Data *p_;
@@ -360,7 +402,12 @@ void BatchSolver::ensure_column(int s, int j) {
   while (sl.chunks.size() < need) {
     void *p = nullptr;
     const size_t bytes = vb_ * L_.ld * ((size_t)1 << L_.chunk_shift);
-    PV_HIP(hipMalloc(&p, bytes));
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();  // clear the error state for the caller
+      throw DeviceOom{bytes};
+    }
+    PV_HIP(e);
     // rows [n, ld) of every column stay zero: the reorthogonalization
     // kernels stream whole row pairs up to ld
     PV_HIP(hipMemsetAsync(p, 0, bytes, c_.stream));
@@ -400,13 +447,20 @@ void BatchSolver::free_basis(int s) {
 void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
   const DevCsr &A = *c_.A;
   const int g = (int)grp.size();
-  // the narrowest kernel width that holds the group (groups never exceed
-  // the tuned width): a width-8 block holding 4 columns would stream twice
-  // the vector bytes it needs
-  int B = 1;
-  while (B < g) B *= 2;
   const bool f32 = c_.filter_prec == PEVSL_PREC_FP32;
   const size_t elem = f32 ? 4 : 8;
+  int kind = block_kind();
+  // the narrowest kernel width that holds the group (groups never exceed
+  // the tuned width): a width-8 block holding 4 columns would stream twice
+  // the vector bytes it needs.  The segmented kernel takes any multiple of
+  // its 16-byte piece.
+  const int seg = 16 / (int)elem;
+  int B = 1;
+  if (kind == 5)
+    B = (g + seg - 1) / seg * seg;
+  else
+    while (B < g) B *= 2;
+  if (kind != 5 && (B > 32 || A.nrowblk == 0)) kind = 0;
   const size_t blk_elems = (size_t)n_ * B;
   char *base = reinterpret_cast<char *>(blk_.ensure((4 * blk_elems * elem) / 8 + 8));
   void *xb[4];
@@ -453,16 +507,10 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
   const double cc = S_[(size_t)grp[0]].pol->cc, dd = S_[(size_t)grp[0]].pol->dd;
   std::vector<int> degs;
   for (int s : grp) degs.push_back(S_[(size_t)s].pol->deg);
-  // block kernel: 1 row per thread, 2 its Clenshaw form, 3 / 4 the
-  // nnz-balanced forms of 1 / 2, 0 lane per (row, column).  Default: the
-  // Clenshaw form on the FP32 path, the reference's three-term recurrence on
-  // FP64, and the balanced forms when some row is long.
-  int kind = c_.blk_rowkernel;
-  if (kind < 0) kind = (f32 ? 2 : 1) + (A.max_row > kUnevenRow ? 2 : 0);
-  if (B > 32 || A.nrowblk == 0) kind = 0;
   const bool rowk = kind != 0;
+  const bool segk = kind == 5;
   const bool clen = kind == 2 || kind == 4;
-  const bool bal = kind >= 3;
+  const bool bal = kind == 3 || kind == 4;
   auto run = [&](auto *tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
     T *x = (T *)xb[0], *y = (T *)xb[1];
@@ -480,7 +528,42 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
     }
     T *yout = y;
     const std::string kname = filter_kernel_name(f32, kind, B, false, 0);
-    if (clen && degmax >= 1) {
+    if (segk) {
+      // pieces of `seg` columns, each active from its own top degree down
+      // (columns are sorted by degree, so a piece's first column has its top)
+      const int np = B / seg;
+      SegTops tops{};
+      for (int p = 0; p < np; p++) tops.top[p] = p * seg < g ? degs[(size_t)p * seg] : -1;
+      T *ring[3] = {y, b[0], b[1]};
+      const T *b1 = nullptr, *b2 = nullptr;
+      int nr = 0;
+      long tok = -1;
+      double kbytes = 0.0;
+      for (int d = degmax; d >= 0; d--) {
+        T *out = ring[nr];
+        if ((const T *)out == b1 || (const T *)out == b2) {
+          nr = (nr + 1) % 3;
+          out = ring[nr];
+        }
+        nr = (nr + 1) % 3;
+        int nseg = 0;  // pieces whose series has reached degree d
+        while (nseg < np && tops.top[nseg] >= d) nseg++;
+        if (tok < 0 && !capture_) tok = c_.kt.begin(c_.stream, kname.c_str());
+        clen_step_seg<T>(A, B, b1, b2, x, out, cc, d == 0 ? 1.0 / dd : 2.0 / dd, mus_.p, d, nseg,
+                         tops, c_.stream);
+        const double lb = clen_seg_launch_bytes(A, (int)elem, seg, d, nseg, tops);
+        kbytes += lb;
+        c_.kstats.filter_bytes += lb;
+        int nprod = 0;  // columns that form a product at this degree
+        while (nprod < g && degs[(size_t)nprod] > d) nprod++;
+        c_.kstats.filter_steps += nprod;
+        c_.kstats.filter_launches++;
+        b2 = b1;
+        b1 = out;
+        yout = out;
+      }
+      if (tok >= 0) c_.kt.end(c_.stream, tok, degmax + 1, kbytes);
+    } else if (clen && degmax >= 1) {
       // Clenshaw sums from the top degree down: b_d = mu_d x + (2/dd)(A - cc)
       // b_{d+1} - b_{d+2}; the d = 0 step (1/dd) is the filtered vector.
       // Three rotating buffers; x (the input block) is read by every step.
@@ -585,10 +668,14 @@ std::vector<std::vector<int>> BatchSolver::filter_groups(const std::vector<int> 
     if (pa->dd != pb->dd) return pa->dd < pb->dd;
     return pa->deg > pb->deg;
   });
-  // group width: the tuned one, else 8 (FP32) / 4 (FP64), which pEVSL_Autotune
-  // picks on the 7-point stencil (wider row gathers cost more per column)
+  // group width: the tuned one, else the widest block of the segmented
+  // kernel (one pass over the matrix for all columns), else 8 (FP32) / 4
+  // (FP64), which pEVSL_Autotune picks for the row kernels on the 7-point
+  // stencil (wider row gathers cost more per column)
+  const int kind = block_kind();
   const int gmax = c_.block_width ? c_.block_width
-                                  : (c_.filter_prec == PEVSL_PREC_FP32 ? 8 : 4);
+                   : kind == 5    ? kMaxSegs * 16 / (c_.filter_prec == PEVSL_PREC_FP32 ? 4 : 8)
+                   : (c_.filter_prec == PEVSL_PREC_FP32 ? 8 : 4);
   std::vector<std::vector<int>> out;
   for (size_t i = 0; i < order.size();) {
     const pevsl_Polparams *p0 = S_[(size_t)order[i]].pol;
@@ -1381,18 +1468,86 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs, int *max
     }
     return 0;
   }
-  BatchSolver bs(c, pevsl, nslices);
-  const int rc = bs.run(intvs, maxits, tol, vinit, pols, fstats);
-  if (rc == -1) {
+  // outputs of every slice released and set empty (an error return)
+  auto clear_outputs = [&](bool release) {
     for (int s = 0; s < nslices; s++) {
+      if (release) {
+        free(lamo[s]);
+        free(reso[s]);
+        if (Wo && Wo[s]) {
+          pEVSL_ParvecsFree(Wo[s]);
+          free(Wo[s]);
+        }
+      }
       nevOut[s] = 0;
       lamo[s] = reso[s] = NULL;
       if (Wo) Wo[s] = NULL;
     }
+  };
+  clear_outputs(false);
+  const int w = c.batch_wave;
+  try {
+    if (w <= 0 || nslices <= w) {
+      BatchSolver bs(c, pevsl, nslices);
+      const int rc = bs.run(intvs, maxits, tol, vinit, pols, fstats);
+      if (rc == -1) return rc;
+      bs.outputs(nevOut, lamo, Wo, reso, vinit);
+      return rc;
+    }
+    // waves of at most w slices (balanced sizes), in order of filter degree
+    // so that the slices of one wave share filter groups of similar degree
+    std::vector<int> order((size_t)nslices);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return pols[a].deg < pols[b].deg; });
+    const int nw = (nslices + w - 1) / w;
+    std::vector<pevsl_LanInfo> info((size_t)nslices, pevsl_LanInfo{});
+    int rc = 0, pos = 0;
+    for (int wi = 0; wi < nw; wi++) {
+      const int m = (nslices - pos + (nw - wi) - 1) / (nw - wi);
+      std::vector<double> iv((size_t)4 * m);
+      std::vector<int> mi((size_t)m), nev((size_t)m);
+      std::vector<pevsl_Polparams> pp((size_t)m);
+      std::vector<double *> lam((size_t)m), res((size_t)m);
+      std::vector<pevsl_Parvecs *> W((size_t)m, nullptr);
+      for (int j = 0; j < m; j++) {
+        const int sidx = order[(size_t)(pos + j)];
+        for (int q = 0; q < 4; q++) iv[(size_t)4 * j + q] = intvs[(size_t)4 * sidx + q];
+        mi[(size_t)j] = maxits[sidx];
+        pp[(size_t)j] = pols[sidx];
+      }
+      int r;
+      {
+        BatchSolver bs(c, pevsl, m);
+        r = bs.run(iv.data(), mi.data(), tol, vinit, pp.data(), fstats);
+        if (r == -1) {
+          clear_outputs(true);
+          return r;
+        }
+        bs.outputs(nev.data(), lam.data(), Wo ? W.data() : nullptr, res.data(), vinit);
+      }
+      for (int j = 0; j < m; j++) {
+        const int sidx = order[(size_t)(pos + j)];
+        nevOut[sidx] = nev[(size_t)j];
+        lamo[sidx] = lam[(size_t)j];
+        reso[sidx] = res[(size_t)j];
+        if (Wo) Wo[sidx] = W[(size_t)j];
+        info[(size_t)sidx] = c.info[(size_t)j];
+      }
+      if (r) rc = r;
+      pos += m;
+    }
+    c.info = info;
     return rc;
+  } catch (const DeviceOom &e) {
+    fprintf(stderr,
+            "pEVSL_ChebLanNrBatch: a Lanczos basis chunk of %zu bytes does not fit in device "
+            "memory; solve fewer slices at once (pEVSL_SetBatchWave)\n",
+            e.bytes);
+    clear_outputs(true);
+    c.info.assign((size_t)nslices, pevsl_LanInfo{});
+    return PEVSL_ERR_DEVICE_OOM;
   }
-  bs.outputs(nevOut, lamo, Wo, reso, vinit);
-  return rc;
 }
 
 int pEVSL_BenchBatchStep(pevsl_Data *pevsl, int nslices, double *intvs, pevsl_Polparams *pols,
@@ -1401,8 +1556,14 @@ int pEVSL_BenchBatchStep(pevsl_Data *pevsl, int nslices, double *intvs, pevsl_Po
   if (!c.A || pevsl->ifGenEv || nslices <= 0 || k < 1 || k + 2 > pevsl->N || warmup < 0 ||
       reps < 0 || graph_reps < 0 || !out)
     return -1;
-  BatchSolver bs(c, pevsl, nslices);
-  return bs.bench(intvs, pols, k, warmup, reps, graph_reps, out);
+  try {
+    BatchSolver bs(c, pevsl, nslices);
+    return bs.bench(intvs, pols, k, warmup, reps, graph_reps, out);
+  } catch (const DeviceOom &e) {
+    fprintf(stderr, "pEVSL_BenchBatchStep: %d bases of %d columns do not fit in device memory\n",
+            nslices, k + 2);
+    return PEVSL_ERR_DEVICE_OOM;
+  }
 }
 
 int pEVSL_ChebAv(pevsl_Data *pevsl, pevsl_Polparams *pol, pevsl_Parvec *v,

@@ -51,7 +51,8 @@ double time_vec(Ctx &c, int lanes, int reps) {
 }
 
 // kind: 0 lane per (row, column), 1 row per thread, 2 row per thread in
-// Clenshaw form, 3 / 4 the nnz-balanced forms of 1 / 2
+// Clenshaw form, 3 / 4 the nnz-balanced forms of 1 / 2, 5 the segmented
+// Clenshaw form (B a multiple of 16 bytes of columns, at most kMaxSegs pieces)
 template <typename T>
 double time_blk(Ctx &c, int B, int reps, DBuf<double> &mus, int kind) {
   const DevCsr &A = *c.A;
@@ -60,8 +61,13 @@ double time_blk(Ctx &c, int B, int reps, DBuf<double> &mus, int kind) {
   b.ensure(4 * blk);
   PV_HIP(hipMemsetAsync(b.p, 0, sizeof(T) * 4 * blk, c.stream));
   T *v0 = b.p, *v1 = b.p + blk, *v2 = b.p + 2 * blk, *y = b.p + 3 * blk;
+  SegTops tops{};
+  for (int p = 0; p < kMaxSegs; p++) tops.top[p] = 100;  // every piece forms the product
+  const int nseg = B * (int)sizeof(T) / 16;
   auto step = [&](const T *vk, const T *vkm1, T *vkp1) {
-    if (kind == 4)
+    if (kind == 5)
+      clen_step_seg<T>(A, B, vk, vkm1, y, vkp1, 6.0, 1.0 / 3, mus.p, 2, nseg, tops, c.stream);
+    else if (kind == 4)
       clen_step_bal<T>(A, B, vk, vkm1, y, vkp1, 6.0, 1.0 / 3, mus.p, 2, c.stream);
     else if (kind == 3)
       cheb_step_bal<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
@@ -131,6 +137,17 @@ int pEVSL_Autotune(pevsl_Data *p, int reps, pevsl_TuneResult *res) {
       bw = B;
       rowk = 0;
     }
+    // segmented Clenshaw kernel (FP32): one pass over the matrix for all
+    // B columns, so its cost per column falls with B
+    if (f32 && B % 4 == 0) {
+      const double ts = time_blk<float>(c, B, reps, mus, 5);
+      res->t_blockseg[i] = ts;
+      if (ts / B < bestc) {
+        bestc = ts / B;
+        bw = B;
+        rowk = 5;
+      }
+    }
     if (B <= 32) {
       const double tr = f32 ? time_blk<float>(c, B, reps, mus, 1)
                             : time_blk<double>(c, B, reps, mus, 1);
@@ -187,13 +204,15 @@ int pEVSL_BenchBlock(pevsl_Data *p, int B, int reps, double *sec, double *bytes)
   // the kernel a batched solve of width B uses (see BatchSolver::filter_block)
   int kind = c.blk_rowkernel;
   if (kind < 0) kind = (f32 ? 2 : 1) + (c.A->max_row > kUnevenRow ? 2 : 0);
-  if (!f32 && (kind == 2 || kind == 4)) return -1;  // Clenshaw forms: FP32 only
+  if (!f32 && (kind == 2 || kind == 4 || kind == 5)) return -1;  // Clenshaw forms: FP32 only
+  if (kind == 5 && B < 4) return -1;  // a whole 16-byte piece of columns
   DBuf<double> mus;
   mus.ensure(3 * 64);
   std::vector<double> mh(3 * 64, 0.01);
   PV_HIP(hipMemcpy(mus.p, mh.data(), sizeof(double) * mh.size(), hipMemcpyHostToDevice));
   *sec = f32 ? time_blk<float>(c, B, reps, mus, kind) : time_blk<double>(c, B, reps, mus, kind);
-  *bytes = (kind == 2 || kind == 4) ? step_bytes(*c.A, elem, B) - (double)c.A->n * B * elem
+  *bytes = (kind == 2 || kind == 4 || kind == 5)
+               ? step_bytes(*c.A, elem, B) - (double)c.A->n * B * elem
                                     : step_bytes(*c.A, elem, B);
   return 0;
 }

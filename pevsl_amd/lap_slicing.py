@@ -40,6 +40,11 @@ def main(argv=None):
     ap.add_argument("--mode", choices=["fp64", "mixed", "mixed-mlp"], default="mixed-mlp")
     ap.add_argument("--wave", type=int, default=0,
                     help="slices per batched call (0: all); the FP32 bases of one wave must fit in HBM")
+    ap.add_argument("--basis", choices=["fp32", "fp64"], default="fp32",
+                    help="Lanczos basis storage of the mixed modes")
+    ap.add_argument("--block-kernel", type=int, default=-1,
+                    help="block SpMM kernel of the batched filter (pEVSL_SetBlockKernel codes; "
+                         "-1: the library's default)")
     ap.add_argument("--weights", default=WEIGHTS, help="learned-slicer weights (mixed-mlp)")
     ap.add_argument("--slices", default="",
                     help="solve only these slices of the slicing, e.g. 0-7 or 0,5,9 (default: all)")
@@ -66,8 +71,9 @@ def main(argv=None):
     if a.mode == "fp64":
         kw.update(prec=capi.PEVSL_PREC_FP64, batch=False)
     else:
-        kw.update(prec=capi.PEVSL_PREC_FP32, basis_prec=capi.PEVSL_PREC_FP32, batch=True,
-                  wave=a.wave or None)
+        kw.update(prec=capi.PEVSL_PREC_FP32, batch=True, wave=a.wave or None,
+                  basis_prec=capi.PEVSL_PREC_FP64 if a.basis == "fp64" else capi.PEVSL_PREC_FP32,
+                  block_rowkernel=a.block_kernel)
         if a.mode == "mixed-mlp":
             kw.update(dostype=3, mlp=a.weights, judge=True)
     g = a.grid
@@ -82,6 +88,7 @@ def main(argv=None):
         "workload": "Laplacian %d^3, [%g, %g] in %d slices" % (g, a.interval[0], a.interval[1],
                                                              a.nslices),
         "mode": a.mode, "wave": a.wave or a.nslices if a.mode != "fp64" else 1,
+        "basis": a.basis if a.mode != "fp64" else "fp64", "block_kernel": a.block_kernel,
         "slices_solved": [d["slice"] for d in sl],
         "wall_s": round(wall, 2), "lanczos_s": round(r["t_lanczos"], 2),
         "dos_slicing_s": round(r["t_dos"], 3), "findpol_s": round(r["t_findpol"], 3),

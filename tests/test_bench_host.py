@@ -23,9 +23,17 @@ def test_cpu_sample_units():
     """20^3 slice [0.6, 0.7]: the oracle sample reports a positive time per
     degree step at the slice's FindPol degree, a Gram-Schmidt time per column
     and the thread count."""
-    smp = bench.cpu_sample(20, 0.6, 0.7, 0, 22)
+    smp = bench.cpu_sample(20, 0.6, 0.7, 0, 22, steps=2, reorthk=128)
     assert smp["deg"] > 1 and smp["t_degree_step"] > 0 and smp["t_cgs_per_col"] > 0
-    assert smp["threads"] >= 1 and smp["steps"] == 2
+    assert smp["threads"] >= 1 and smp["steps"] == 2 and smp["reorthk"] == 128
+
+
+def test_cpu_model_counts_filter_and_gram_schmidt():
+    """(steps + 1) filter applications at the degree, steps^2 / 2 Gram-Schmidt
+    columns, summed over slices."""
+    smp = {"t_degree_step": 2e-3, "t_cgs_per_col": 1e-4}
+    t = bench.cpu_model(smp, [(100, 10), (200, 20)])
+    assert abs(t - (11 * 100 * 2e-3 + 1e-4 * 50 + 21 * 200 * 2e-3 + 1e-4 * 200)) < 1e-12
 
 
 def test_fp64_model_reproduces_its_inputs():

@@ -129,11 +129,13 @@ def test_c3_mixed_batched_two_slices(lib, lap100, c3_slices, basis):
             tr = capi.TuneResult()
             assert lib.pEVSL_Autotune(inst.p, 10, C.byref(tr)) == 0
             assert tr.block_width in (2, 4, 8, 16, 32, 64)
-            assert tr.block_rowkernel in (0, 1, 2, 3, 4)
+            assert tr.block_rowkernel in (0, 1, 2, 3, 4, 5)
+            assert all(tr.t_blockseg[j] > 0 for j in range(1, 6))  # widths 4..64
             i = tr.block_width.bit_length() - 2
-            assert tr.t_blockclen[i] > 0 and tr.bytes_clen[i] > 0
+            t_sel = tr.t_blockseg[i] if tr.block_rowkernel == 5 else tr.t_blockclen[i]
+            assert t_sel > 0 and tr.bytes_clen[min(i, 4)] > 0
             print("autotune: width %d kernel %d, %.2f us per step" % (
-                tr.block_width, tr.block_rowkernel, 1e6 * tr.t_blockclen[i]), flush=True)
+                tr.block_width, tr.block_rowkernel, 1e6 * t_sel), flush=True)
         ev_int = int(1 + ecnt / 20.0)
         mlan = min(max(5 * (ev_int + 2), 300), lap100.n)
         vinit = driver.rand_vinit(lib, lap100.n)
@@ -146,7 +148,8 @@ def test_c3_mixed_batched_two_slices(lib, lap100, c3_slices, basis):
             nev = check_slice(o, sli[s], sli[s + 1], o["info"]["tol_accept"])
             print("slice %d: %d eigenvalues, its %d" % (s, nev, o["info"]["its"]), flush=True)
         kt = inst.kernel_times()
-        assert any(k.startswith("k_clen_blkrow<float,") for k in kt), kt
+        assert any(k.startswith("k_clen_blkrow<float,") or k.startswith("k_clen_seg<float,")
+                   for k in kt), kt
     finally:
         inst.close()
         for p in pols:

@@ -324,20 +324,67 @@ def test_tridiag_multisection_against_lapack(lib, n):
     assert abs(tr.value - sel.sum()) <= 64 * np.finfo(float).eps * scale * np.sqrt(n) * max(1, len(sel))
 
 
-@pytest.mark.parametrize("batch,rowk", [(False, -1), (True, 1), (True, 2), (True, 3), (True, 4)])
+@pytest.mark.parametrize("batch,rowk", [(False, -1), (True, 1), (True, 2), (True, 3), (True, 4),
+                                        (True, 5)])
 def test_mixed_precision_c1(lib, batch, rowk):
     banner("C1 mixed precision (FP32 filter + FP64 Rayleigh-Ritz) batch=%s block kernel %d"
            % (batch, rowk))
     # stated mixed-path tolerance: residual < 1e-6 * rho (rho = 12 here),
     # eigenvalue error < 1e-10;
     # block kernel 1 = three-term recurrence, 2 = Clenshaw form, 3 / 4 their
-    # nnz-balanced forms
+    # nnz-balanced forms, 5 = segmented Clenshaw form (all 5 slices in one
+    # block of two 4-column pieces)
     r = driver.lap_slicing_run(20, 20, 20, 0.6, 1.2, 5, batch=batch, prec=capi.PEVSL_PREC_FP32,
                                block_rowkernel=rowk)
     for s in r["slices"]:
         assert s["nev"] == s["nev_exact"]
         assert s["max_rel_err"] < 1e-10
         assert np.all(s["res"] < 1.2e-5)
+
+
+@pytest.mark.parametrize("basis", [capi.PEVSL_PREC_FP64, capi.PEVSL_PREC_FP32])
+def test_segmented_clenshaw_equals_row_kernel(lib, basis):
+    """Block kernel 5 (one thread per (row, 4-column piece), every piece
+    active from its own top degree) against kernel 2 (one thread per row,
+    all columns of the block every degree) on 6 slices of uneven degree in
+    one block of width 8 (pieces of 4 + 2 columns, 2 padding): each column
+    goes through the same operations in the same order, so the two solves
+    agree bit for bit (eigenvalues, their order, step counts)."""
+    banner("segmented vs row Clenshaw kernel, 16^3, 6 slices, basis %d" % basis)
+    g = 16
+    ia, ja, a = driver.laplacian3d(g, g, g)
+    M = driver.Matrix(lib, ia, ja, a)
+    edges = np.linspace(0.4, 1.6, 7)
+    intvs = [[edges[i], edges[i + 1], 0.0, 12.0] for i in range(6)]
+    pols = [driver.find_pol(lib, iv)[0] for iv in intvs]
+    degs = [p_.deg for p_ in pols]
+    assert len(set(degs)) == 6
+    vinit = driver.rand_vinit(lib, M.n)
+    runs = {}
+    try:
+        for kind in (2, 5):
+            inst = driver.Instance(lib, M, prec=capi.PEVSL_PREC_FP32, basis_prec=basis,
+                                   block_rowkernel=kind)
+            try:
+                rc, outs = driver.cheblan_batch(inst, intvs, [400] * 6, 1e-8, vinit, pols)
+                assert rc == 0
+                runs[kind] = (outs, inst.kernel_times(), inst.kernel_stats())
+            finally:
+                inst.close()
+    finally:
+        for p_ in pols:
+            lib.pEVSL_FreePol(C.byref(p_))
+        M.free()
+    assert "k_clen_seg<float,8>" in runs[5][1], runs[5][1].keys()
+    assert "k_clen_blkrow<float,8>" in runs[2][1], runs[2][1].keys()
+    # both count every column's products once per degree
+    assert runs[5][2]["filter_steps"] == runs[2][2]["filter_steps"]
+    for iv, o2, o5 in zip(intvs, runs[2][0], runs[5][0]):
+        assert o2["info"]["its"] == o5["info"]["its"]
+        np.testing.assert_array_equal(o5["lam"], o2["lam"])
+        ex = orc.exact_eig_lap3(g, g, g, iv[0], iv[1])
+        assert len(o5["lam"]) == len(ex) > 0
+        np.testing.assert_allclose(np.sort(o5["lam"]), ex, rtol=1e-10)
 
 
 @pytest.mark.parametrize("batch", [False, True])
@@ -495,7 +542,8 @@ def test_kpmdos_uneven_rows_against_oracle(lib, prec):
     (capi.PEVSL_PREC_FP64, capi.PEVSL_PREC_FP64, -1),
     (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, -1),
     (capi.PEVSL_PREC_FP64, capi.PEVSL_PREC_FP64, 1),
-    (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, 2)])
+    (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, 2),
+    (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, 5)])
 def test_irregular_small_counts_match_dense(lib, prec, basis, rowk):
     """The irregular-sparsity recipe (power-law rows, Lanczos bounds, a 5%
     share of the spectrum, 16 KPM slices) at n = 3000, where the dense
@@ -504,7 +552,8 @@ def test_irregular_small_counts_match_dense(lib, prec, basis, rowk):
     (184 eigenvalues).  Rows reach ~2000 entries, so the default (-1) runs
     the nnz-balanced block kernels with their whole-workgroup long-row path
     (16 columns: rows over 256 FP64 / 512 FP32 entries); 1 and 2 are the
-    row-per-thread kernels."""
+    row-per-thread kernels; 5 the segmented Clenshaw kernel, whose row
+    blocks of 2048/Q nonzeros leave the longest rows unstaged."""
     banner("irregular n=3000, 16 slices, prec %d basis %d, block kernel %d, batched"
            % (prec, basis, rowk))
     r = driver.irregular_slicing_run(3000, nslices=16, seed=11, nvec=64, center=0.2, prec=prec,
@@ -686,7 +735,8 @@ def test_cheb_av_empty_rows_against_oracle(lib, prec):
     (capi.PEVSL_PREC_FP64, capi.PEVSL_PREC_FP64, 3, True),
     (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, -1, False),
     (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, 2, True),
-    (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, 4, True)])
+    (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, 4, True),
+    (capi.PEVSL_PREC_FP32, capi.PEVSL_PREC_FP32, 5, True)])
 def test_cheblan_empty_rows_counts_match_dense(lib, prec, basis, rowk, batch):
     """ChebLanNr on the matrix with empty rows, one slice at a time (stream
     kernel) or as a batch of two (row kernels 1 / 2, nnz-balanced kernels
