@@ -272,10 +272,19 @@ def probe(kind):
         inst = driver.Instance(lib, M, prec=capi.PEVSL_PREC_FP32, basis_prec=capi.PEVSL_PREC_FP32)
         try:
             tr = capi.TuneResult()
-            lib.pEVSL_Autotune(inst.p, 5, C.byref(tr))
+            lib.pEVSL_Autotune(inst.p, 20, C.byref(tr))
             lib.pEVSL_BenchReorth(inst.p, 1024, 3, C.byref(sec), C.byref(byt))
             log("probe cgs32 %.3f ms/pass; tuned width %d kernel %d" % (
                 sec.value * 1e3, tr.block_width, tr.block_rowkernel))
+            # the autotuner's table: us per launch (per column) of each block kernel
+            fmt = lambda t, B: "%8.2f (%5.2f)" % (t * 1e6, t * 1e6 / B) if t > 0 else "%16s" % "-"  # noqa: E731
+            log("probe autotune, FP32, us per launch (per column): B | Clenshaw row | segmented "
+                "Clenshaw | three-term row | balanced Clenshaw")
+            for i in range(6):
+                B = 2 << i
+                log("probe   %2d | %s | %s | %s | %s" % (
+                    B, fmt(tr.t_blockclen[i], B), fmt(tr.t_blockseg[i], B),
+                    fmt(tr.t_blockrow[i], B), fmt(tr.t_blockbalclen[i], B)))
         finally:
             inst.close()
     finally:
