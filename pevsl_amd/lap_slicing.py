@@ -37,6 +37,8 @@ def main(argv=None):
     ap.add_argument("--grid", type=int, default=100, help="nx = ny = nz")
     ap.add_argument("--interval", type=float, nargs=2, default=[0.6, 1.2])
     ap.add_argument("--nslices", type=int, default=20)
+    ap.add_argument("--nvec", type=int, default=60,
+                    help="random vectors of the KPM (TESTS/Lap/LapPLanN.c uses 60)")
     ap.add_argument("--mode", choices=["fp64", "mixed", "mixed-mlp"], default="mixed-mlp")
     ap.add_argument("--wave", type=int, default=0,
                     help="slices per batched call (0: all); the FP32 bases of one wave must fit in HBM")
@@ -67,7 +69,7 @@ def main(argv=None):
         if not a.quiet:
             print(m, file=sys.stderr, flush=True)
 
-    kw = dict(nvec=60, Mdeg=300, log=log, slices_only=only, verbose=a.verbose)
+    kw = dict(nvec=a.nvec, Mdeg=300, log=log, slices_only=only, verbose=a.verbose)
     if a.mode == "fp64":
         kw.update(prec=capi.PEVSL_PREC_FP64, batch=False)
     else:
