@@ -314,6 +314,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe", choices=["lap", "irregular"], default=None)
+    ap.add_argument("--graph-filter", action="store_true",
+                    help="filter groups of the solve and of the timed steps replayed from hipGraphs "
+                         "(pEVSL_SetGraphFilter)")
     ap.add_argument("--graph-reps", type=int, default=20,
                     help="replays of the captured filter group in the hipGraph-vs-eager "
                          "comparison (0: no comparison)")
@@ -348,7 +351,8 @@ def main():
                                   "autotuned" % (g, g ** 3, a, b, ns, args.nvec,
                                                  "waves of %d slices" % wave if wave else
                                                  "one wave"),
-                      "basis": args.basis, "wave": wave, "parallelism": "1 GPU",
+                      "basis": args.basis, "wave": wave, "graph_filter": bool(args.graph_filter),
+                      "parallelism": "1 GPU",
                       "step": "one iteration of the batched solve (the slices of one wave) at a "
                               "fixed basis size"}}
     em = Emitter(out)
@@ -371,6 +375,7 @@ def main():
     n = M.n
     inst = driver.Instance(lib, M, prec=capi.PEVSL_PREC_FP32, basis_prec=basis)
     lib.pEVSL_SetBatchWave(inst.p, wave)
+    lib.pEVSL_SetGraphFilter(inst.p, 1 if args.graph_filter else 0)
     tune = capi.TuneResult()
     lib.pEVSL_Autotune(inst.p, 20, C.byref(tune))
     net = slc.Mlp.load(slc.DEFAULT_WEIGHTS, lib)

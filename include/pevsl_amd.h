@@ -166,6 +166,14 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs,
  * or 4-byte FP32 elements).  Outputs are per slice as for one batch. */
 int pEVSL_SetBatchWave(pevsl_Data *pevsl, int wave);
 
+/* Filter groups of pEVSL_ChebLanNrBatch replayed from hipGraphs (on = 1;
+ * default 0: eager launches).  The first step of each group captures the
+ * kernels of one filter application (pack, one launch per degree, unpack)
+ * with the group's own coefficients and column pointers on the device; later
+ * steps restage the pointers and replay the graph.  Results are bitwise those
+ * of eager launches. */
+int pEVSL_SetGraphFilter(pevsl_Data *pevsl, int on);
+
 /* Return code of pEVSL_ChebLanNrBatch / pEVSL_ChebLanNr when a Lanczos
  * basis does not fit in device memory; the call releases what it held and
  * returns no eigenpairs (nevOut 0, NULL outputs). */

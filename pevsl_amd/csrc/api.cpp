@@ -380,6 +380,12 @@ int pEVSL_SetSpmvTuning(pevsl_Data *p, int lanes, int bw) {
   return 0;
 }
 
+int pEVSL_SetGraphFilter(pevsl_Data *p, int on) {
+  if (on != 0 && on != 1) return -1;
+  ctx_of(p).graph_filter = on;
+  return 0;
+}
+
 int pEVSL_SetBatchWave(pevsl_Data *p, int wave) {
   if (wave < 0) return -1;
   ctx_of(p).batch_wave = wave;

@@ -130,6 +130,7 @@ struct Ctx {
   int keep_vectors = 1;
   int chunk_cols = 512;            // Lanczos basis allocation granularity
   int batch_wave = 0;              // ChebLanNrBatch: slices per wave (0: all at once)
+  int graph_filter = 0;            // batched filter groups replayed from hipGraphs
   std::vector<pevsl_LanInfo> info; // per slice of the last (batched) solve
   pevsl_KernelStats kstats{};      // accumulated device timings
   KernTimers kt;                   // per-kernel HIP-event timings

@@ -35,6 +35,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <type_traits>
 
@@ -267,6 +268,33 @@ void gram_tn(rocblas_handle h, hipStream_t st, int n, int m1, const double *A, i
   sum_parts(wk, parts, blk, C, st);
 }
 
+// One filter group replayed from a captured hipGraph (pEVSL_SetGraphFilter):
+// the group keeps its own coefficients and column pointers on the device, so
+// that the graphs of the groups of one step replay one after the other, and
+// the graph holds kernel launches only.
+struct GroupGraph {
+  std::vector<int> key;  // slices, width, kernel
+  DBuf<double> mus;
+  DBuf<void *> colptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  char *blocks = nullptr;  // filter-block allocation the graph was captured on
+  double bytes = 0.0;      // algorithmic bytes of one replay
+  long long steps = 0, launches = 0;
+  std::string kname;
+  bool used = false;
+  GroupGraph() = default;
+  GroupGraph(const GroupGraph &) = delete;
+  GroupGraph &operator=(const GroupGraph &) = delete;
+  void drop() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~GroupGraph() { drop(); }
+};
+
 // Raised when a basis chunk cannot be allocated: the batch does not fit in
 // device memory (pEVSL_ChebLanNrBatch returns PEVSL_ERR_DEVICE_OOM).
 struct DeviceOom {
@@ -320,6 +348,13 @@ class BatchSolver {
   void set_active(const std::vector<int> &act);
   void filter_step(const std::vector<int> &act, int k);
   void filter_block(const std::vector<int> &grp, int k);
+  int group_width(int g, int &kind) const;
+  char *blocks_for(int B);
+  void stage_mus(const std::vector<int> &grp, int B, DBuf<double> &dst);
+  void stage_colptr(const std::vector<int> &grp, int B, int k, DBuf<void *> &dst);
+  void enqueue_block(const std::vector<int> &grp, int B, int kind, const double *musp,
+                     void *const *colp);
+  void filter_group_graph(const std::vector<int> &grp, int k);
   void lanczos_update(const std::vector<int> &act, int k);
   void cgs(const std::vector<int> &act, int k);
   void breakdown(int s, int k);
@@ -370,6 +405,7 @@ class BatchSolver {
   std::vector<int> mus_key_;
   std::vector<double> mus_h_, betas_h_;
   std::vector<void *> cp_h_;
+  std::vector<std::unique_ptr<GroupGraph>> graphs_;  // captured filter groups
 };
 
 const double *BatchSolver::chunk64(int s, int ci, int kc) {
@@ -444,66 +480,94 @@ void BatchSolver::free_basis(int s) {
 
 // Block filter for a group of at most B slices (sorted by degree, highest
 // first): z_{k+1} = p_s(A) z_k for every slice s of the group.
-void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
-  const DevCsr &A = *c_.A;
-  const int g = (int)grp.size();
-  const bool f32 = c_.filter_prec == PEVSL_PREC_FP32;
-  const size_t elem = f32 ? 4 : 8;
-  int kind = block_kind();
-  // the narrowest kernel width that holds the group (groups never exceed
-  // the tuned width): a width-8 block holding 4 columns would stream twice
-  // the vector bytes it needs.  The segmented kernel takes any multiple of
-  // its 16-byte piece.
-  const int seg = 16 / (int)elem;
+// Width and kernel of the block that filters a group of g slices: the
+// narrowest width that holds the group (groups never exceed the tuned width:
+// a width-8 block holding 4 columns would stream twice the vector bytes it
+// needs); the segmented kernel takes any multiple of its 16-byte piece.
+int BatchSolver::group_width(int g, int &kind) const {
+  const int seg = c_.filter_prec == PEVSL_PREC_FP32 ? 4 : 2;
+  kind = block_kind();
   int B = 1;
   if (kind == 5)
     B = (g + seg - 1) / seg * seg;
   else
     while (B < g) B *= 2;
-  if (kind != 5 && (B > 32 || A.nrowblk == 0)) kind = 0;
+  if (kind != 5 && (B > 32 || c_.A->nrowblk == 0)) kind = 0;
+  return B;
+}
+
+// Interleaved filter blocks (x, y and two work blocks) for width B; returns
+// the base of the allocation (it moves when a wider block needs more room).
+char *BatchSolver::blocks_for(int B) {
+  const size_t elem = c_.filter_prec == PEVSL_PREC_FP32 ? 4 : 8;
   const size_t blk_elems = (size_t)n_ * B;
-  char *base = reinterpret_cast<char *>(blk_.ensure((4 * blk_elems * elem) / 8 + 8));
-  void *xb[4];
-  for (int i = 0; i < 4; i++) xb[i] = base + (size_t)i * blk_elems * elem;
-  // per-column coefficients [(degmax+1) x B]
+  return reinterpret_cast<char *>(blk_.ensure((4 * blk_elems * elem) / 8 + 8));
+}
+
+// Per-column coefficients [(degmax+1) x B] of a group into dst (host staging
+// rewritten only after the stream has drained the previous copy).
+void BatchSolver::stage_mus(const std::vector<int> &grp, int B, DBuf<double> &dst) {
   int degmax = 0;
-  std::vector<int> key;
-  for (int s : grp) {
-    degmax = std::max(degmax, S_[(size_t)s].pol->deg);
-    key.push_back(s);
+  for (int s : grp) degmax = std::max(degmax, S_[(size_t)s].pol->deg);
+  PV_HIP(hipStreamSynchronize(c_.stream));
+  std::vector<double> &mh = mus_h_;
+  mh.assign((size_t)(degmax + 1) * B, 0.0);
+  for (size_t ci = 0; ci < grp.size(); ci++) {
+    const pevsl_Polparams *pol = S_[(size_t)grp[ci]].pol;
+    for (int j = 0; j <= pol->deg; j++) mh[(size_t)j * B + ci] = pol->mu[j];
   }
+  dst.ensure(mh.size());
+  PV_HIP(hipMemcpyAsync(dst.p, mh.data(), sizeof(double) * mh.size(), hipMemcpyHostToDevice,
+                        c_.stream));
+}
+
+// Column pointers of a group at step k: inputs z_k, outputs z_{k+1}.
+void BatchSolver::stage_colptr(const std::vector<int> &grp, int B, int k, DBuf<void *> &dst) {
+  PV_HIP(hipStreamSynchronize(c_.stream));
+  std::vector<void *> &cp = cp_h_;
+  cp.assign((size_t)2 * B, nullptr);
+  for (size_t ci = 0; ci < grp.size(); ci++) {
+    cp[ci] = col(grp[ci], k);
+    cp[(size_t)B + ci] = col(grp[ci], k + 1);
+  }
+  dst.ensure(cp.size());
+  PV_HIP(hipMemcpyAsync(dst.p, cp.data(), sizeof(void *) * cp.size(), hipMemcpyHostToDevice,
+                        c_.stream));
+}
+
+void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
+  int kind;
+  const int B = group_width((int)grp.size(), kind);
+  std::vector<int> key(grp);
   key.push_back(B);
-  // host staging (mus_h_, cp_h_) is rewritten only after the per-step
-  // synchronization, when earlier copies have completed
   if (key != mus_key_) {
     if (capture_) fatal("filter_block: coefficients not staged before capture", __FILE__,
                         __LINE__);
-    PV_HIP(hipStreamSynchronize(c_.stream));
-    std::vector<double> &mh = mus_h_;
-    mh.assign((size_t)(degmax + 1) * B, 0.0);
-    for (int ci = 0; ci < g; ci++) {
-      const pevsl_Polparams *pol = S_[(size_t)grp[(size_t)ci]].pol;
-      for (int j = 0; j <= pol->deg; j++) mh[(size_t)j * B + ci] = pol->mu[j];
-    }
-    mus_.ensure(mh.size());
-    PV_HIP(hipMemcpyAsync(mus_.p, mh.data(), sizeof(double) * mh.size(),
-                          hipMemcpyHostToDevice, c_.stream));
+    stage_mus(grp, B, mus_);
     mus_key_ = key;
   }
-  // column pointers: inputs z_k, outputs z_{k+1} (under graph capture the
-  // pointers of this group and step are already on the device)
-  if (!capture_) {
-    PV_HIP(hipStreamSynchronize(c_.stream));
-    std::vector<void *> &cp = cp_h_;
-    cp.assign((size_t)2 * B, nullptr);
-    for (int ci = 0; ci < g; ci++) {
-      cp[(size_t)ci] = col(grp[(size_t)ci], k);
-      cp[(size_t)B + ci] = col(grp[(size_t)ci], k + 1);
-    }
-    colptr_.ensure(cp.size());
-    PV_HIP(hipMemcpyAsync(colptr_.p, cp.data(), sizeof(void *) * cp.size(),
-                          hipMemcpyHostToDevice, c_.stream));
-  }
+  // under graph capture the pointers of this group and step are already on
+  // the device
+  if (!capture_) stage_colptr(grp, B, k, colptr_);
+  enqueue_block(grp, B, kind, mus_.p, colptr_.p);
+}
+
+// The kernels of one filter application to a group (pack, the degree loop,
+// unpack), with coefficients musp and column pointers colp on the device:
+// nothing but kernel launches, so that it can be captured in a hipGraph.
+void BatchSolver::enqueue_block(const std::vector<int> &grp, int B, int kind,
+                                const double *musp, void *const *colp) {
+  const DevCsr &A = *c_.A;
+  const int g = (int)grp.size();
+  const bool f32 = c_.filter_prec == PEVSL_PREC_FP32;
+  const size_t elem = f32 ? 4 : 8;
+  const int seg = 16 / (int)elem;
+  const size_t blk_elems = (size_t)n_ * B;
+  char *base = blocks_for(B);
+  void *xb[4];
+  for (int i = 0; i < 4; i++) xb[i] = base + (size_t)i * blk_elems * elem;
+  int degmax = 0;
+  for (int s : grp) degmax = std::max(degmax, S_[(size_t)s].pol->deg);
   const double cc = S_[(size_t)grp[0]].pol->cc, dd = S_[(size_t)grp[0]].pol->dd;
   std::vector<int> degs;
   for (int s : grp) degs.push_back(S_[(size_t)s].pol->deg);
@@ -517,13 +581,13 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
     T *b[3] = {(T *)xb[2], (T *)xb[3], nullptr};
     if constexpr (std::is_same<T, float>::value) {
       if (b32_)
-        pack_cols<float, float>(n_, B, g, reinterpret_cast<const float *const *>(colptr_.p), x,
+        pack_cols<float, float>(n_, B, g, reinterpret_cast<const float *const *>(colp), x,
                                 c_.stream);
       else
-        pack_cols<float, double>(n_, B, g, reinterpret_cast<const double *const *>(colptr_.p),
+        pack_cols<float, double>(n_, B, g, reinterpret_cast<const double *const *>(colp),
                                  x, c_.stream);
     } else {
-      pack_cols<T, double>(n_, B, g, reinterpret_cast<const double *const *>(colptr_.p), x,
+      pack_cols<T, double>(n_, B, g, reinterpret_cast<const double *const *>(colp), x,
                            c_.stream);
     }
     T *yout = y;
@@ -549,7 +613,7 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
         int nseg = 0;  // pieces whose series has reached degree d
         while (nseg < np && tops.top[nseg] >= d) nseg++;
         if (tok < 0 && !capture_) tok = c_.kt.begin(c_.stream, kname.c_str());
-        clen_step_seg<T>(A, B, b1, b2, x, out, cc, d == 0 ? 1.0 / dd : 2.0 / dd, mus_.p, d, nseg,
+        clen_step_seg<T>(A, B, b1, b2, x, out, cc, d == 0 ? 1.0 / dd : 2.0 / dd, musp, d, nseg,
                          tops, c_.stream);
         const double lb = clen_seg_launch_bytes(A, (int)elem, seg, d, nseg, tops);
         kbytes += lb;
@@ -583,10 +647,10 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
         nr = (nr + 1) % 3;
         if (b1 && tok < 0 && !capture_) tok = c_.kt.begin(c_.stream, kname.c_str());
         if (bal)
-          clen_step_bal<T>(A, B, b1, b2, x, out, cc, d == 0 ? 1.0 / dd : 2.0 / dd, mus_.p, d,
+          clen_step_bal<T>(A, B, b1, b2, x, out, cc, d == 0 ? 1.0 / dd : 2.0 / dd, musp, d,
                            c_.stream);
         else
-          clen_step_blkrow<T>(A, B, b1, b2, x, out, cc, d == 0 ? 1.0 / dd : 2.0 / dd, mus_.p,
+          clen_step_blkrow<T>(A, B, b1, b2, x, out, cc, d == 0 ? 1.0 / dd : 2.0 / dd, musp,
                               d, c_.stream);
         int nact = 0;  // columns whose series has reached degree d + 1
         while (nact < g && degs[(size_t)nact] >= d + 1) nact++;
@@ -620,14 +684,14 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
       }
       nb = (nb + 1) % 3;
       if (bal)
-        cheb_step_bal<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd, mus_.p, d,
+        cheb_step_bal<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd, musp, d,
                          nact, c_.stream);
       else if (rowk)
         cheb_step_blkrow<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd,
-                            mus_.p, d, nact, c_.stream);
+                            musp, d, nact, c_.stream);
       else
         cheb_step_blk<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd,
-                         mus_.p, d, nact, c_.stream);
+                         musp, d, nact, c_.stream);
       const double lb = (double)filter_launch_bytes(A, (int)elem, nact, d == 1);
       kbytes += lb;
       c_.kstats.filter_bytes += lb;
@@ -641,12 +705,12 @@ void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
     if constexpr (std::is_same<T, float>::value) {
       if (b32_)
         unpack_cols<float, float>(n_, B, g, yout,
-                                  reinterpret_cast<float *const *>(colptr_.p + B), c_.stream);
+                                  reinterpret_cast<float *const *>(colp + B), c_.stream);
       else
         unpack_cols<float, double>(n_, B, g, yout,
-                                   reinterpret_cast<double *const *>(colptr_.p + B), c_.stream);
+                                   reinterpret_cast<double *const *>(colp + B), c_.stream);
     } else {
-      unpack_cols<T, double>(n_, B, g, yout, reinterpret_cast<double *const *>(colptr_.p + B),
+      unpack_cols<T, double>(n_, B, g, yout, reinterpret_cast<double *const *>(colp + B),
                              c_.stream);
     }
   };
@@ -699,14 +763,82 @@ void BatchSolver::filter_step(const std::vector<int> &act, int k) {
     }
     return;
   }
-  for (const std::vector<int> &grp : filter_groups(act)) {
+  const std::vector<std::vector<int>> groups = filter_groups(act);
+  const bool graphs = c_.graph_filter != 0;
+  if (graphs) {
+    // the blocks sized for the widest group first, so that no allocation
+    // moves them between a capture and its replays within the step
+    int bmax = 1;
+    for (const std::vector<int> &grp : groups) {
+      int kind;
+      if (grp.size() > 1) bmax = std::max(bmax, group_width((int)grp.size(), kind));
+    }
+    blocks_for(bmax);
+    for (auto &gg : graphs_) gg->used = false;
+  }
+  for (const std::vector<int> &grp : groups) {
     if (grp.size() == 1) {
       filter_into(grp[0], nullptr, k, k + 1);
+    } else if (graphs) {
+      filter_group_graph(grp, k);
     } else {
       filter_block(grp, k);
     }
     for (int s : grp) S_[(size_t)s].nmatvec += S_[(size_t)s].pol->deg;
   }
+  if (graphs) {
+    // groups only shrink as slices finish: a graph unused this step is not
+    // replayed again
+    graphs_.erase(std::remove_if(graphs_.begin(), graphs_.end(),
+                                 [](const std::unique_ptr<GroupGraph> &g) { return !g->used; }),
+                  graphs_.end());
+  }
+}
+
+// Filter a group by replaying its captured hipGraph; the first use of a
+// group (or of a new filter-block allocation) captures the kernels of one
+// application.  Per step only the column pointers are restaged.
+void BatchSolver::filter_group_graph(const std::vector<int> &grp, int k) {
+  int kind;
+  const int B = group_width((int)grp.size(), kind);
+  std::vector<int> key(grp);
+  key.push_back(B);
+  key.push_back(kind);
+  GroupGraph *gg = nullptr;
+  for (auto &g : graphs_)
+    if (g->key == key) gg = g.get();
+  if (!gg) {
+    graphs_.push_back(std::unique_ptr<GroupGraph>(new GroupGraph()));
+    gg = graphs_.back().get();
+    gg->key = key;
+    gg->kname = filter_kernel_name(c_.filter_prec == PEVSL_PREC_FP32, kind, B, false, 0);
+    stage_mus(grp, B, gg->mus);
+  }
+  gg->used = true;
+  stage_colptr(grp, B, k, gg->colptr);
+  char *base = blocks_for(B);
+  if (gg->exec && gg->blocks != base) gg->drop();
+  if (!gg->exec) {
+    const pevsl_KernelStats ks = c_.kstats;
+    capture_ = true;
+    PV_HIP(hipStreamBeginCapture(c_.stream, hipStreamCaptureModeThreadLocal));
+    enqueue_block(grp, B, kind, gg->mus.p, gg->colptr.p);
+    PV_HIP(hipStreamEndCapture(c_.stream, &gg->graph));
+    capture_ = false;
+    PV_HIP(hipGraphInstantiate(&gg->exec, gg->graph, nullptr, nullptr, 0));
+    gg->blocks = base;
+    // the capture counted one application that has not run
+    gg->bytes = c_.kstats.filter_bytes - ks.filter_bytes;
+    gg->steps = c_.kstats.filter_steps - ks.filter_steps;
+    gg->launches = c_.kstats.filter_launches - ks.filter_launches;
+    c_.kstats = ks;
+  }
+  const long tok = c_.kt.begin(c_.stream, gg->kname.c_str());
+  PV_HIP(hipGraphLaunch(gg->exec, c_.stream));
+  c_.kt.end(c_.stream, tok, gg->launches, gg->bytes);
+  c_.kstats.filter_bytes += gg->bytes;
+  c_.kstats.filter_steps += gg->steps;
+  c_.kstats.filter_launches += gg->launches;
 }
 
 // three-term update: alpha and |z_{k+1}|^2 before reorthogonalization

@@ -6,7 +6,9 @@ return of pEVSL_ChebLanNrBatch.
   in the caller's slice order;
 * a batch whose Lanczos bases do not fit in device memory returns
   PEVSL_ERR_DEVICE_OOM with empty outputs instead of ending the process
-  (run in a child process that first fills the device memory).
+  (run in a child process that first fills the device memory);
+* filter groups replayed from hipGraphs (pEVSL_SetGraphFilter) give the
+  eager solve bit for bit.
 """
 import ctypes as C
 import json
@@ -128,3 +130,56 @@ def test_batch_out_of_device_memory_returns_error(lib):
     assert "does not fit in device memory" in out.stderr
     assert r["rc2"] == 0
     assert r["nev2"] == len(orc.exact_eig_lap3(40, 40, 40, 0.6, 0.8)) > 0
+
+
+@pytest.mark.parametrize("prec,kind", [(capi.PEVSL_PREC_FP32, 2), (capi.PEVSL_PREC_FP32, 5),
+                                       (capi.PEVSL_PREC_FP64, 1)])
+def test_graph_filter_equals_eager(lib, prec, kind):
+    """pEVSL_SetGraphFilter: every step replays each filter group from a
+    hipGraph captured on the group's first step, with the solver's own eager
+    kernels and synchronizations between the replays; groups change as
+    slices finish, which captures new graphs.  The solve must equal the
+    eager one bit for bit (eigenvalues, their order, step counts)."""
+    banner("batched solve, filter groups from hipGraphs vs eager, prec %d kernel %d"
+           % (prec, kind))
+    g = 16
+    ia, ja, a = driver.laplacian3d(g, g, g)
+    M = driver.Matrix(lib, ia, ja, a)
+    edges = np.linspace(0.4, 1.6, 7)
+    intvs = [[edges[i], edges[i + 1], 0.0, 12.0] for i in range(6)]
+    pols = [driver.find_pol(lib, iv)[0] for iv in intvs]
+    vinit = driver.rand_vinit(lib, M.n)
+    runs = {}
+    try:
+        for graphs in (0, 1):
+            inst = driver.Instance(lib, M, prec=prec, basis_prec=prec, block_rowkernel=kind)
+            try:
+                assert lib.pEVSL_SetGraphFilter(inst.p, graphs) == 0
+                rc, outs = driver.cheblan_batch(inst, intvs, [400] * 6, 1e-8, vinit, pols)
+                assert rc == 0
+                runs[graphs] = (outs, inst.kernel_stats(), inst.kernel_times())
+            finally:
+                inst.close()
+        inst = driver.Instance(lib, M)
+        try:
+            assert lib.pEVSL_SetGraphFilter(inst.p, 2) == -1
+        finally:
+            inst.close()
+    finally:
+        for p_ in pols:
+            lib.pEVSL_FreePol(C.byref(p_))
+        M.free()
+    (eager, ks0, kt0), (graph, ks1, kt1) = runs[0], runs[1]
+    for iv, o0, o1 in zip(intvs, eager, graph):
+        assert o0["info"]["its"] == o1["info"]["its"]
+        np.testing.assert_array_equal(o1["lam"], o0["lam"])
+        ex = orc.exact_eig_lap3(g, g, g, iv[0], iv[1])
+        assert len(o1["lam"]) == len(ex) > 0
+    # the replays are counted like the eager launches
+    assert ks1["filter_steps"] == ks0["filter_steps"]
+    assert ks1["filter_launches"] == ks0["filter_launches"]
+    assert abs(ks1["filter_bytes"] - ks0["filter_bytes"]) <= 1e-9 * ks0["filter_bytes"]
+    blk = [k_ for k_ in kt0 if "blk" in k_ or "clen" in k_]
+    assert blk and all(k_ in kt1 for k_ in blk)
+    for k_ in blk:
+        assert kt1[k_]["launches"] == kt0[k_]["launches"]
