@@ -160,8 +160,9 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs,
 
 /* Slices per wave of pEVSL_ChebLanNrBatch (0, the default: all slices in
  * one batch).  With wave > 0 a call with more slices solves them in
- * ceil(nslices / wave) batches of balanced size, in order of filter degree,
- * one after the other: peak device memory is then that of the largest wave
+ * ceil(nslices / wave) batches of `wave` slices (the last takes the rest),
+ * in order of filter degree, one after the other: peak device memory is
+ * then that of the largest wave
  * (each slice's Lanczos basis grows to n x its step count, in 8-byte FP64
  * or 4-byte FP32 elements).  Outputs are per slice as for one batch. */
 int pEVSL_SetBatchWave(pevsl_Data *pevsl, int wave);
@@ -185,6 +186,15 @@ int pEVSL_SetGraphFilter(pevsl_Data *pevsl, int on);
  * callers that apply a designed filter outside the Lanczos solve. */
 int pEVSL_ChebAv(pevsl_Data *pevsl, pevsl_Polparams *pol, pevsl_Parvec *v,
                  pevsl_Parvec *y, pevsl_Parvec *w);
+
+/* Classical Gram-Schmidt of v against Q(:, 0:k-1) with the DGKS test (at
+ * most i_max passes; another pass while the new norm is at most 1/sqrt(2)
+ * of the previous one), *nrmv = |v| after the last pass; w unused.  Same
+ * arguments as the reference's internal CGS_DGKS (INC/pevsl_int.h:37,
+ * SRC/miscla.c:211-240), on host vectors; the work runs in the device
+ * kernels of the Lanczos solve's reorthogonalization (FP64). */
+int pEVSL_CGS_DGKS(pevsl_Data *pevsl, int k, int i_max, pevsl_Parvecs *Q, pevsl_Parvec *v,
+                   double *nrmv, double *w);
 
 /* Diagnostics of slice s of the last ChebLanNr / ChebLanNrBatch call. */
 int pEVSL_GetLanInfo(pevsl_Data *pevsl, int s, pevsl_LanInfo *info);

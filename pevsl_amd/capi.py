@@ -313,6 +313,8 @@ SIGNATURES = {
     "pEVSL_ChebAvNoAlloc": (C.c_int, [DataP, C.POINTER(Polparams), C.c_void_p, C.c_void_p]),
     "pEVSL_SetBatchWave": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetGraphFilter": (C.c_int, [DataP, C.c_int]),
+    "pEVSL_CGS_DGKS": (C.c_int, [DataP, C.c_int, C.c_int, C.POINTER(Parvecs), C.POINTER(Parvec),
+                                 C.POINTER(C.c_double), C.c_void_p]),
     "pEVSL_ChebAvNoAllocMaxRows": (C.c_size_t, []),
     "pEVSL_DeviceInfo": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_size_t)]),
     "pEVSL_SymmTridEigvals": (C.c_int, [C.c_int, c_double_p, c_double_p, C.c_double,

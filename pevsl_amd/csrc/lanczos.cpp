@@ -1626,8 +1626,9 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs, int *max
       bs.outputs(nevOut, lamo, Wo, reso, vinit);
       return rc;
     }
-    // waves of at most w slices (balanced sizes), in order of filter degree
-    // so that the slices of one wave share filter groups of similar degree
+    // waves of w slices (the last takes the rest), in order of filter degree
+    // so that the slices of one wave share filter groups of similar degree;
+    // full waves keep the filter groups at their full width
     std::vector<int> order((size_t)nslices);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
@@ -1636,7 +1637,7 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs, int *max
     std::vector<pevsl_LanInfo> info((size_t)nslices, pevsl_LanInfo{});
     int rc = 0, pos = 0;
     for (int wi = 0; wi < nw; wi++) {
-      const int m = (nslices - pos + (nw - wi) - 1) / (nw - wi);
+      const int m = std::min(w, nslices - pos);
       std::vector<double> iv((size_t)4 * m);
       std::vector<int> mi((size_t)m), nev((size_t)m);
       std::vector<pevsl_Polparams> pp((size_t)m);

@@ -298,4 +298,81 @@ int pEVSL_BenchReorth(pevsl_Data *p, int k, int reps, double *sec, double *bytes
   return 0;
 }
 
+// CGS_DGKS on host vectors (INC/pevsl_int.h:37, SRC/miscla.c:211-240): the
+// reorthogonalization kernels of the Lanczos solve (GEMV-T and GEMV-N over a
+// device basis, FP64 sums) on Q(:, 0:k-1) and v, with the reference's DGKS
+// loop: after each pass the new norm is compared with 1/sqrt(2) of the old
+// one on the device, and another pass runs (at most i_max in all) while it
+// is not larger.
+int pEVSL_CGS_DGKS(pevsl_Data *p, int k, int i_max, pevsl_Parvecs *Q, pevsl_Parvec *v,
+                   double *nrmv, double *w) {
+  (void)w;
+  Ctx &c = ctx_of(p);
+  const int n = p->n;
+  if (k < 0 || i_max < 1 || !v || v->n_local != n || (k > 0 && (!Q || Q->n_local != n ||
+                                                                   Q->n_vecs < k)))
+    return -4;
+  const double t0 = wtime();
+  LanBatch L;
+  L.n = n;
+  L.ld = ((size_t)n + 63) / 64 * 64;
+  L.nslice = 1;
+  int shift = 5;
+  while ((1 << shift) < k + 1) shift++;
+  L.chunk_shift = shift;
+  L.maxchunks = 1;
+  L.vbytes = 8;  // the reference's Q and v are FP64
+  const size_t cnt = L.ld * ((size_t)1 << shift);
+  DBuf<double> V, sc, h, part, pt;
+  DBuf<void *> vt;
+  DBuf<int> fl;
+  V.ensure(cnt);
+  fill_zero_f64(cnt, V.p, c.stream);  // rows [n, ld) stay zero
+  if (k > 0)
+    PV_HIP(hipMemcpy2DAsync(V.p, L.ld * sizeof(double), Q->data, (size_t)Q->ld * sizeof(double),
+                            (size_t)n * sizeof(double), (size_t)k, hipMemcpyHostToDevice,
+                            c.stream));
+  double *z = V.p + (size_t)k * L.ld;
+  PV_HIP(hipMemcpyAsync(z, v->data, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+  void *vp = V.p;
+  L.vtab = vt.ensure(1);
+  PV_HIP(hipMemcpyAsync(L.vtab, &vp, sizeof(void *), hipMemcpyHostToDevice, c.stream));
+  double *s = sc.ensure(4);
+  L.alpha = s;
+  L.beta = s + 1;
+  L.nrm2a = s + 2;
+  L.nrm2b = s + 3;
+  int *f = fl.ensure(2);
+  L.pass2 = f;
+  L.active = f + 1;
+  const int flags[2] = {0, 1};
+  PV_HIP(hipMemcpyAsync(f, flags, sizeof(int) * 2, hipMemcpyHostToDevice, c.stream));
+  L.hcap = k + 1;
+  L.h = h.ensure((size_t)L.hcap);
+  L.part_cap = lan_part_doubles(n, 1, L.hcap);
+  L.part = part.ensure(L.part_cap);
+  // |v|^2 before the first pass
+  pt.ensure((size_t)dot_part_blocks(n) + 8);
+  const int nb = dot_partials(n, z, z, pt.p, c.stream);
+  reduce_partials(pt.p, nb, 1, L.nrm2a, c.stream);
+  double nrm2 = 0.0;
+  if (k == 0) {
+    PV_HIP(hipMemcpyAsync(&nrm2, L.nrm2a, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  }
+  for (int it = 0; k > 0 && it < i_max; it++) {
+    lan_cgs_pass(L, k - 1, 0, c.stream);  // basis columns 0..k-1, vector in column k
+    int again = 0;
+    PV_HIP(hipMemcpyAsync(&nrm2, L.nrm2b, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    PV_HIP(hipMemcpyAsync(&again, L.pass2, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    PV_HIP(hipStreamSynchronize(c.stream));
+    if (!again) break;  // new norm > old / sqrt(2)
+    PV_HIP(hipMemcpyAsync(L.nrm2a, L.nrm2b, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  }
+  PV_HIP(hipMemcpyAsync(v->data, z, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+  PV_HIP(hipStreamSynchronize(c.stream));
+  if (nrmv) *nrmv = sqrt(nrm2);
+  p->stats->t_reorth += wtime() - t0;
+  return 0;
+}
+
 }  // extern "C"

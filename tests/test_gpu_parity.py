@@ -290,6 +290,44 @@ def test_reorth_kernel_orthogonality(lib):
     assert np.max(np.abs(G - np.eye(len(Y)))) < 1e-8
 
 
+@pytest.mark.parametrize("n,k,i_max,near", [(5000, 40, 2, 0.0), (5000, 100, 2, 1e-9),
+                                             (3001, 33, 1, 1e-9), (4096, 0, 2, 0.0),
+                                             (20000, 300, 3, 1e-12)])
+def test_cgs_dgks_against_oracle(lib, n, k, i_max, near):
+    """pEVSL_CGS_DGKS (the device reorthogonalization kernels of the Lanczos
+    solve, CGS_DGKS of SRC/miscla.c:211-240) against the oracle's restatement
+    on the same Q and v.  `near` > 0 puts v within that distance of span(Q),
+    so that the first pass removes almost all of it and the DGKS test asks
+    for another pass; widths k = 33, 100 and 300 leave partial 32-column
+    groups in the GEMV-T kernel; k = 0 leaves v as it is."""
+    banner("CGS_DGKS n=%d k=%d i_max=%d near=%g" % (n, k, i_max, near))
+    rng = np.random.default_rng(n + k)
+    Q = np.linalg.qr(rng.standard_normal((n, max(k, 1))))[0][:, :k] if k else np.zeros((n, 0))
+    Q = np.asfortranarray(Q)
+    r = rng.standard_normal(n)
+    v = Q @ rng.standard_normal(k) + near * r if near > 0 else r
+    inst = driver.Instance(lib)
+    try:
+        lib.pEVSL_SetProbSizes(inst.p, n, n, 0)
+        Qp = capi.Parvecs()
+        lib.pEVSL_ParvecsCreateShell(n, max(k, 1), n, n, 0, 0, C.byref(Qp),
+                                     Q.ctypes.data_as(capi.c_double_p) if k else None)
+        Qp.n_vecs = k
+        vg = np.ascontiguousarray(v.copy())
+        nrm = C.c_double()
+        assert lib.pEVSL_CGS_DGKS(inst.p, k, i_max, C.byref(Qp) if k else None,
+                                  C.byref(parvec(lib, vg)), C.byref(nrm), None) == 0
+    finally:
+        inst.close()
+    vo, nrm_o = orc.cgs_dgks(Q, v, i_max)
+    scale = np.linalg.norm(v)
+    np.testing.assert_allclose(vg, vo, rtol=0, atol=1e-13 * scale)
+    assert abs(nrm.value - nrm_o) <= 1e-12 * scale
+    if k:
+        # what the passes leave of span(Q): rounding of the last pass
+        assert np.max(np.abs(Q.T @ vg)) <= 1e-13 * scale
+
+
 @pytest.mark.parametrize("n", [1, 2, 7, 300, 2000])
 def test_tridiag_multisection_against_lapack(lib, n):
     banner("GPU Sturm multisection n=%d" % n)
