@@ -196,6 +196,19 @@ int pEVSL_ChebAv(pevsl_Data *pevsl, pevsl_Polparams *pol, pevsl_Parvec *v,
 int pEVSL_CGS_DGKS(pevsl_Data *pevsl, int k, int i_max, pevsl_Parvecs *Q, pevsl_Parvec *v,
                    double *nrmv, double *w);
 
+/* Gram products G = U^T U and H = U^T A U of the FP64 Rayleigh-Ritz stage
+ * (mixed path): 0 = rocBLAS strided-batched DGEMM over row parts,
+ * 1 = the library's own FP64 MFMA kernel (v_mfma_f64_16x16x4 tiles over row
+ * parts, summed in a fixed order), -1 (default) = the MFMA kernel below
+ * 65,536 rows, where rocBLAS would run one unsplit DGEMM, rocBLAS above. */
+int pEVSL_SetGramKernel(pevsl_Data *pevsl, int kernel);
+
+/* Times both Gram kernels on C = X^T Y for random n x m FP64 X, Y (HIP
+ * events, `reps` calls each) and returns the largest difference of the two
+ * results relative to the largest entry. */
+int pEVSL_BenchGram(pevsl_Data *pevsl, int n, int m, int reps, double *sec_blas,
+                    double *sec_mfma, double *maxdiff);
+
 /* Diagnostics of slice s of the last ChebLanNr / ChebLanNrBatch call. */
 int pEVSL_GetLanInfo(pevsl_Data *pevsl, int s, pevsl_LanInfo *info);
 int pEVSL_GetKernelStats(pevsl_Data *pevsl, pevsl_KernelStats *st);

@@ -313,6 +313,9 @@ SIGNATURES = {
     "pEVSL_ChebAvNoAlloc": (C.c_int, [DataP, C.POINTER(Polparams), C.c_void_p, C.c_void_p]),
     "pEVSL_SetBatchWave": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetGraphFilter": (C.c_int, [DataP, C.c_int]),
+    "pEVSL_SetGramKernel": (C.c_int, [DataP, C.c_int]),
+    "pEVSL_BenchGram": (C.c_int, [DataP, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pEVSL_CGS_DGKS": (C.c_int, [DataP, C.c_int, C.c_int, C.POINTER(Parvecs), C.POINTER(Parvec),
                                  C.POINTER(C.c_double), C.c_void_p]),
     "pEVSL_ChebAvNoAllocMaxRows": (C.c_size_t, []),

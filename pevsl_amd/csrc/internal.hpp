@@ -131,6 +131,8 @@ struct Ctx {
   int chunk_cols = 512;            // Lanczos basis allocation granularity
   int batch_wave = 0;              // ChebLanNrBatch: slices per wave (0: all at once)
   int graph_filter = 0;            // batched filter groups replayed from hipGraphs
+  int gram_kernel = -1;            // Rayleigh-Ritz Gram products: 0 rocBLAS, 1 MFMA kernel,
+                                   // -1 by size
   std::vector<pevsl_LanInfo> info; // per slice of the last (batched) solve
   pevsl_KernelStats kstats{};      // accumulated device timings
   KernTimers kt;                   // per-kernel HIP-event timings

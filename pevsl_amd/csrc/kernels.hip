@@ -1066,6 +1066,94 @@ __global__ void k_unpack(int n, int B, int ncol, const T *__restrict__ blk,
 }
 
 // ---------------------------------------------------------------------------
+// Tall-skinny Gram products on the FP64 matrix cores
+// ---------------------------------------------------------------------------
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// part[p] (m1 x m2, column-major) = X(rows of part p, :)^T Y(rows of part p, :)
+// for the row parts [p*chunk, min(n, (p+1)*chunk)).  One workgroup per
+// (T x T tile of C, row part), T = 32 WB.  KR rows of the tile's T columns of
+// X and of Y are staged in LDS per step (rows padded by one double, so that
+// the column-wise stores and the row-wise MFMA reads fall in distinct
+// banks); the next step's rows are loaded into registers while the matrix
+// cores work on the current ones.  Each of the 4 waves accumulates a
+// (T/2) x (T/2) quarter of the tile as WB x WB blocks of v_mfma_f64_16x16x4:
+// lane l supplies A(l % 16, l / 16) = X(r + l/16, i + l%16) and
+// B(l / 16, l % 16) = Y(r + l/16, j + l%16), and holds C(l/16 + 4q, l%16) of
+// its block in accumulator q.  The rows of a part are summed in order, so the
+// partial products are deterministic.
+template <int WB, int KR>
+__global__ void __launch_bounds__(kThreads)
+k_gram_mfma(int n, int m1, int m2, const double *__restrict__ X, size_t ldx,
+            const double *__restrict__ Y, size_t ldy, int chunk, double *__restrict__ part) {
+  constexpr int T = 32 * WB;
+  constexpr int PER = KR * T / kThreads;  // staged elements per thread and matrix
+  __shared__ double Xs[KR][T + 1];
+  __shared__ double Ys[KR][T + 1];
+  const int tiles1 = (m1 + T - 1) / T;
+  const int i0 = (int)(blockIdx.x % tiles1) * T;
+  const int j0 = (int)(blockIdx.x / tiles1) * T;
+  const int r0 = (int)blockIdx.y * chunk;
+  const int r1 = min(n, r0 + chunk);
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wi = (T / 2) * (wave & 1), wj = (T / 2) * (wave >> 1);
+  v4d acc[WB][WB];
+#pragma unroll
+  for (int a = 0; a < WB; a++)
+#pragma unroll
+    for (int b = 0; b < WB; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+  // element e = t + q*kThreads of a step: row e % KR, column e / KR (KR
+  // consecutive threads read KR consecutive rows of one column)
+  double xr[PER], yr[PER];
+  auto fetch = [&](int r) {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = t + q * kThreads, rr = e % KR, cc = e / KR, row = r + rr;
+      xr[q] = (row < r1 && i0 + cc < m1) ? X[(size_t)(i0 + cc) * ldx + row] : 0.0;
+      yr[q] = (row < r1 && j0 + cc < m2) ? Y[(size_t)(j0 + cc) * ldy + row] : 0.0;
+    }
+  };
+  if (r0 < r1) fetch(r0);
+  for (int r = r0; r < r1; r += KR) {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = t + q * kThreads;
+      Xs[e % KR][e / KR] = xr[q];
+      Ys[e % KR][e / KR] = yr[q];
+    }
+    __syncthreads();
+    if (r + KR < r1) fetch(r + KR);  // in flight while the matrix cores run
+#pragma unroll
+    for (int kk = 0; kk < KR; kk += 4) {
+      const int kr = kk + lane / 16, c16 = lane % 16;
+      double av[WB], bv[WB];
+#pragma unroll
+      for (int a = 0; a < WB; a++) av[a] = Xs[kr][wi + 16 * a + c16];
+#pragma unroll
+      for (int b = 0; b < WB; b++) bv[b] = Ys[kr][wj + 16 * b + c16];
+#pragma unroll
+      for (int a = 0; a < WB; a++)
+#pragma unroll
+        for (int b = 0; b < WB; b++)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  double *P = part + (size_t)blockIdx.y * m1 * m2;
+#pragma unroll
+  for (int a = 0; a < WB; a++)
+#pragma unroll
+    for (int b = 0; b < WB; b++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int row = i0 + wi + 16 * a + lane / 16 + 4 * q;
+        const int col = j0 + wj + 16 * b + lane % 16;
+        if (row < m1 && col < m2) P[(size_t)col * m1 + row] = acc[a][b][q];
+      }
+}
+
+// ---------------------------------------------------------------------------
 // Lanczos (batched over slices; blockIdx.y = slice)
 // ---------------------------------------------------------------------------
 
@@ -1958,6 +2046,18 @@ int dot_partials(int n, const double *x, const double *y, double *part,
   hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(kThreads), 0, s, n, x, y,
                      part);
   return nb;
+}
+
+void gram_tn_mfma(int n, int m1, const double *X, size_t ldx, int m2, const double *Y,
+                  size_t ldy, int chunk, int nparts, double *part, hipStream_t s) {
+  if (n <= 0 || m1 <= 0 || m2 <= 0 || nparts <= 0) return;
+  // 64 x 64 workgroup tiles, 64-row steps: measured on n = 1e6 rows, m = 800
+  // (FP64), 29.0 ms against 30.0 / 31.0 ms for 32- / 16-row steps and
+  // 33.3 ms for 128 x 128 tiles
+  constexpr int T = 64;
+  const int tiles = ((m1 + T - 1) / T) * ((m2 + T - 1) / T);
+  hipLaunchKernelGGL((k_gram_mfma<2, 64>), dim3(tiles, nparts), dim3(kThreads), 0, s, n, m1, m2,
+                     X, ldx, Y, ldy, chunk, part);
 }
 
 void scale_vec(int n, double *x, const double *inv_scale_dev, double hostscale,

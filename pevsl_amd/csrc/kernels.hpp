@@ -133,6 +133,11 @@ void reduce_partials(const double *part, int nblk, int ncol, double *out,
                      hipStream_t s);
 // y = x for n doubles (n even, 16-byte aligned): plain copy kernel
 void copy_f64(size_t n, const double *x, double *y, hipStream_t s);
+// part[p] (m1 x m2, column-major) = X(rows p*chunk .. , :)^T Y(same rows, :)
+// for p < nparts (the last part ends at row n), FP64 MFMA (hand-written
+// v_mfma_f64_16x16x4 tiles); X and Y column-major with leading dims ldx, ldy.
+void gram_tn_mfma(int n, int m1, const double *X, size_t ldx, int m2, const double *Y,
+                  size_t ldy, int chunk, int nparts, double *part, hipStream_t s);
 // out[e] = sum_p part[p*nel + e] over p = 0..nparts-1 in order
 void sum_parts(const double *part, int nparts, size_t nel, double *out, hipStream_t s);
 // part[blk] = sum x_i*y_i over the block's rows; returns nblk

@@ -245,8 +245,23 @@ namespace {
 // products are then summed in a fixed order.
 constexpr int kGramParts = 64;
 void gram_tn(rocblas_handle h, hipStream_t st, int n, int m1, const double *A, int lda, int m2,
-             const double *B, int ldb, double *C, DBuf<double> &work) {
+             const double *B, int ldb, double *C, DBuf<double> &work, int kernel) {
   const double one = 1.0, zero = 0.0;
+  // default (-1): the library's MFMA kernel where the row split below would
+  // fall back to one unsplit DGEMM (n < 64 x 1024: 6.4 ms against 0.11 ms at
+  // n = 40,000, m = 130), rocBLAS above (at n = 1e6, m = 800: 23.1 ms
+  // against 29.0 ms)
+  if (kernel < 0) kernel = n < kGramParts * 1024 ? 1 : 0;
+  if (kernel == 1) {
+    // hand-written FP64 MFMA tiles over at most kGramParts row parts of >= 1024 rows
+    const int chunk = std::max(1024, (n + kGramParts - 1) / kGramParts);
+    const int parts = (n + chunk - 1) / chunk;
+    const size_t blk = (size_t)m1 * m2;
+    double *wk = work.ensure(blk * parts);
+    gram_tn_mfma(n, m1, A, (size_t)lda, m2, B, (size_t)ldb, chunk, parts, wk, st);
+    sum_parts(wk, parts, blk, C, st);
+    return;
+  }
   const int chunk = n / kGramParts;
   if (chunk < 1024) {
     PV_BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, m1, m2, n,
@@ -1105,8 +1120,8 @@ void BatchSolver::ritz_rr(int s, double tol) {
   ework.ensure((size_t)nsel);
   {
     DBuf<double> gw;
-    gram_tn(c_.blas, st, n_, nsel, U.p, (int)ld, nsel, U.p, (int)ld, G.p, gw);
-    gram_tn(c_.blas, st, n_, nsel, U.p, (int)ld, nsel, W.p, (int)ld, H.p, gw);
+    gram_tn(c_.blas, st, n_, nsel, U.p, (int)ld, nsel, U.p, (int)ld, G.p, gw, c_.gram_kernel);
+    gram_tn(c_.blas, st, n_, nsel, U.p, (int)ld, nsel, W.p, (int)ld, H.p, gw, c_.gram_kernel);
   }
   PV_BLAS(rocsolver_dsygvd(c_.blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper,
                            nsel, H.p, nsel, G.p, nsel, th.p, ework.p, info.p));
@@ -1681,6 +1696,47 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs, int *max
     c.info.assign((size_t)nslices, pevsl_LanInfo{});
     return PEVSL_ERR_DEVICE_OOM;
   }
+}
+
+int pEVSL_SetGramKernel(pevsl_Data *pevsl, int kernel) {
+  if (kernel < -1 || kernel > 1) return -1;
+  ctx_of(pevsl).gram_kernel = kernel;
+  return 0;
+}
+
+int pEVSL_BenchGram(pevsl_Data *pevsl, int n, int m, int reps, double *sec_blas,
+                    double *sec_mfma, double *maxdiff) {
+  Ctx &c = ctx_of(pevsl);
+  if (n < 1 || m < 1 || reps < 1 || !sec_blas || !sec_mfma || !maxdiff) return -1;
+  const size_t ld = ((size_t)n + 63) / 64 * 64;
+  DBuf<double> X, Y, C0, C1, work;
+  X.ensure(ld * m);
+  Y.ensure(ld * m);
+  rand_fill(ld * m, X.p, 11, 0, 0, c.stream);
+  rand_fill(ld * m, Y.p, 13, 0, 0, c.stream);
+  C0.ensure((size_t)m * m);
+  C1.ensure((size_t)m * m);
+  double *out[2] = {C0.p, C1.p};
+  double *sec[2] = {sec_blas, sec_mfma};
+  for (int kern = 0; kern < 2; kern++) {
+    gram_tn(c.blas, c.stream, n, m, X.p, (int)ld, m, Y.p, (int)ld, out[kern], work, kern);
+    EvTimer ev;
+    ev.start(c.stream);
+    for (int r = 0; r < reps; r++)
+      gram_tn(c.blas, c.stream, n, m, X.p, (int)ld, m, Y.p, (int)ld, out[kern], work, kern);
+    ev.stop(c.stream);
+    *sec[kern] = ev.seconds() / reps;
+  }
+  std::vector<double> h0((size_t)m * m), h1((size_t)m * m);
+  PV_HIP(hipMemcpy(h0.data(), C0.p, sizeof(double) * h0.size(), hipMemcpyDeviceToHost));
+  PV_HIP(hipMemcpy(h1.data(), C1.p, sizeof(double) * h1.size(), hipMemcpyDeviceToHost));
+  double dmax = 0.0, cmax = 0.0;
+  for (size_t i = 0; i < h0.size(); i++) {
+    dmax = std::max(dmax, std::fabs(h0[i] - h1[i]));
+    cmax = std::max(cmax, std::fabs(h0[i]));
+  }
+  *maxdiff = cmax > 0 ? dmax / cmax : dmax;
+  return 0;
 }
 
 int pEVSL_BenchBatchStep(pevsl_Data *pevsl, int nslices, double *intvs, pevsl_Polparams *pols,

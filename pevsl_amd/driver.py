@@ -536,9 +536,10 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
                     sli_override=None, block_width=0, log=None, verbose=False,
                     maxit_cap=None, lanes=0, ritz_refine=-1, mixed_tol=None,
                     block_rowkernel=-1, mlp=None, judge=True, judge_tol=0.15, wave=None,
-                    basis_prec=capi.PEVSL_PREC_FP64, lanczos_bounds=False):
+                    basis_prec=capi.PEVSL_PREC_FP64, lanczos_bounds=False, gram_kernel=-1):
     """End-to-end run of the TESTS/Lap/LapPLanN.c flow on the GPU.  Returns a
     dict with the DOS, slices, per-slice filters, eigenvalues and timings.
+    gram_kernel selects the Rayleigh-Ritz Gram products (pEVSL_SetGramKernel).
 
     dostype 0: KPM (Mdeg, nvec) + SpslicerKpm; 1: Lanczos DOS + SpslicerLan;
     3: learned slicer (`mlp`: a slicer.Mlp or a weights path).  With
@@ -562,6 +563,7 @@ def lap_slicing_run(nx, ny, nz, a, b, nslices, nvec=60, Mdeg=300, dostype=0,
                     block_width=block_width, lanes=lanes, ritz_refine=ritz_refine,
                     mixed_tol=mixed_tol, block_rowkernel=block_rowkernel,
                     basis_prec=basis_prec)
+    lib.pEVSL_SetGramKernel(inst.p, gram_kernel)
     out = {"n": n, "nnz": M.nnz, "t_setup": time.time() - t0}
     try:
         if lanczos_bounds:

@@ -328,6 +328,43 @@ def test_cgs_dgks_against_oracle(lib, n, k, i_max, near):
         assert np.max(np.abs(Q.T @ vg)) <= 1e-13 * scale
 
 
+@pytest.mark.parametrize("n,m", [(1000, 7), (40000, 130), (100003, 65), (250000, 100),
+                                 (1000000, 800)])
+def test_gram_mfma_against_rocblas(lib, n, m):
+    """The library's FP64 MFMA Gram kernel (v_mfma_f64_16x16x4 tiles, row
+    parts summed in a fixed order) against the rocBLAS strided-batched DGEMM
+    path on C = X^T Y for random X, Y: partial 64 x 64 tiles (m = 7, 65, 100,
+    130), row counts that leave short parts and partial 64-row steps.  The
+    two summation orders differ in rounding only."""
+    banner("Gram X^T Y, MFMA kernel vs rocBLAS, n=%d m=%d" % (n, m))
+    inst = driver.Instance(lib)
+    try:
+        lib.pEVSL_SetProbSizes(inst.p, n, n, 0)
+        tb, tm, diff = C.c_double(), C.c_double(), C.c_double()
+        assert lib.pEVSL_BenchGram(inst.p, n, m, 3, C.byref(tb), C.byref(tm), C.byref(diff)) == 0
+    finally:
+        inst.close()
+    flops = 2.0 * n * m * m
+    print("rocBLAS %.3f ms (%.1f TFLOP/s), MFMA kernel %.3f ms (%.1f TFLOP/s), max diff %.2e" % (
+        tb.value * 1e3, flops / tb.value / 1e12, tm.value * 1e3, flops / tm.value / 1e12,
+        diff.value), flush=True)
+    assert diff.value < 1e-12
+
+
+@pytest.mark.parametrize("gram_kernel", [0, 1])
+def test_mixed_c1_gram_kernels(lib, gram_kernel):
+    """The mixed path's FP64 Rayleigh-Ritz stage with either Gram kernel
+    (rocBLAS, the MFMA kernel): C1 batched, every slice exact, eigenvalues to
+    1e-10, residuals below 1e-6 * rho."""
+    banner("C1 mixed precision, Rayleigh-Ritz Gram products on kernel %d" % gram_kernel)
+    r = driver.lap_slicing_run(20, 20, 20, 0.6, 1.2, 5, batch=True, prec=capi.PEVSL_PREC_FP32,
+                               gram_kernel=gram_kernel)
+    for s_ in r["slices"]:
+        assert s_["nev"] == s_["nev_exact"]
+        assert s_["max_rel_err"] < 1e-10
+        assert np.all(s_["res"] < 1.2e-5)
+
+
 @pytest.mark.parametrize("n", [1, 2, 7, 300, 2000])
 def test_tridiag_multisection_against_lapack(lib, n):
     banner("GPU Sturm multisection n=%d" % n)
