@@ -285,6 +285,16 @@ def probe(kind):
                 log("probe   %2d | %s | %s | %s | %s" % (
                     B, fmt(tr.t_blockclen[i], B), fmt(tr.t_blockseg[i], B),
                     fmt(tr.t_blockrow[i], B), fmt(tr.t_blockbalclen[i], B)))
+            # the Clenshaw row kernel with 16-bit column offsets and with the
+            # 32-bit indices
+            lib.pEVSL_SetBlockKernel(inst.p, 2)
+            for col16 in (1, 0):
+                lib.pEVSL_SetColumnOffsets16(inst.p, col16)
+                for B in (4, 8):
+                    assert lib.pEVSL_BenchBlock(inst.p, B, 50, C.byref(sec), C.byref(byt)) == 0
+                    log("probe clen row kernel width %d, %s: %.2f us, %.1f MB, %.2f TB/s" % (
+                        B, "16-bit offsets" if col16 else "32-bit indices", sec.value * 1e6,
+                        byt.value / 1e6, byt.value / sec.value / 1e12))
         finally:
             inst.close()
     finally:
@@ -320,6 +330,9 @@ def main():
     ap.add_argument("--graph-reps", type=int, default=20,
                     help="replays of the captured filter group in the hipGraph-vs-eager "
                          "comparison (0: no comparison)")
+    ap.add_argument("--no-col16", action="store_true",
+                    help="32-bit column indices in the filter kernels (default: 16-bit offsets "
+                         "where the matrix has them, pEVSL_SetColumnOffsets16)")
     ap.add_argument("--progress", action="store_true",
                     help="the solver's convergence-test lines (ChebLanNr fstats) on stderr")
     args = ap.parse_args()
@@ -375,7 +388,9 @@ def main():
     ia, ja, av = driver.laplacian3d(g, g, g)
     M = driver.Matrix(lib, ia, ja, av)
     n = M.n
-    inst = driver.Instance(lib, M, prec=capi.PEVSL_PREC_FP32, basis_prec=basis)
+    inst = driver.Instance(lib, M, prec=capi.PEVSL_PREC_FP32, basis_prec=basis,
+                           col16=not args.no_col16)
+    out["config"]["col16"] = bool(lib.pEVSL_ColumnOffsets16(inst.p))
     lib.pEVSL_SetBatchWave(inst.p, wave)
     lib.pEVSL_SetGraphFilter(inst.p, 1 if args.graph_filter else 0)
     tune = capi.TuneResult()
@@ -481,6 +496,15 @@ def main():
                                    for k_, v in kt.items()}}
         filt = {k_: v for k_, v in kt.items() if k_.startswith("k_")}
         roof = roofline_of(filt, traffic)
+        if roof and out["config"]["col16"] and roof["kernel"].startswith(
+                ("k_clen_blkrow", "k_cheb_blkrow", "k_cheb_stream")):
+            # the kernel read 2-byte column offsets: `achieved` counts them;
+            # the same launches against the CSR count with 4-byte indices
+            per = roof["us_per_launch"] * 1e-6
+            b32 = roof["bytes_per_launch"] + 2.0 * int(ia[-1])
+            roof["index_bytes"] = 2
+            roof["achieved_csr32"] = round(b32 / per / 1e9, 1)
+            roof["frac_csr32"] = round(b32 / per / 1e9 / HBM_PEAK_GBS, 4)
         if roof:
             roof["traffic_source"] = tsrc
             roof["measured"] = ("HIP events around each run of back-to-back launches of the kernel "
@@ -512,7 +536,7 @@ def main():
         need = 1.3 * sum(21e-6 * degs[s] * its[s] * (n / 1e6) + 1.25e-6 * its[s] ** 2 * (n / 1e6)
                          for s in sub) + 15
         if not args.no_fp64 and left() > need:
-            inst64 = driver.Instance(lib, M, prec=capi.PEVSL_PREC_FP64)
+            inst64 = driver.Instance(lib, M, prec=capi.PEVSL_PREC_FP64, col16=not args.no_col16)
             t64 = capi.TuneResult()
             lib.pEVSL_Autotune(inst64.p, 20, C.byref(t64))
             lib.pEVSL_SetSpmvTuning(inst64.p, t64.lanes, 1)

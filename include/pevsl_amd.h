@@ -175,6 +175,19 @@ int pEVSL_SetBatchWave(pevsl_Data *pevsl, int wave);
  * of eager launches. */
 int pEVSL_SetGraphFilter(pevsl_Data *pevsl, int on);
 
+/* 16-bit column offsets (no counterpart in the reference).  At
+ * pEVSL_ParcsrSetup each nonzero's column is also stored as a 16-bit offset
+ * from the first row of its row block (at most 256 rows and 2,048 nonzeros),
+ * when every offset fits (banded orderings: the natural order of a grid
+ * Laplacian; random sparsity does not fit).  The CSR-stream and row-per-
+ * thread block kernels then read 2 index bytes per nonzero instead of 4;
+ * results are bitwise the same.  on = 1 (default) uses them where the
+ * matrix has them, 0 keeps the 32-bit indices for this instance.
+ * pEVSL_ColumnOffsets16 returns 1 when the instance's matrix is read
+ * through them. */
+int pEVSL_SetColumnOffsets16(pevsl_Data *pevsl, int on);
+int pEVSL_ColumnOffsets16(pevsl_Data *pevsl);
+
 /* Return code of pEVSL_ChebLanNrBatch / pEVSL_ChebLanNr when a Lanczos
  * basis does not fit in device memory; the call releases what it held and
  * returns no eigenpairs (nevOut 0, NULL outputs). */

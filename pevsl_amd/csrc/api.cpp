@@ -292,9 +292,12 @@ int pEVSL_SetAParcsr(pevsl_Data *p, pevsl_Parcsr *A) {
   if (!p->Amv) PEVSL_CALLOC(p->Amv, 1, pevsl_Matvec);
   p->Amv->func = parcsr_mv_host;
   p->Amv->data = (void *)A;
-  c.A = dev_matrix_of(A);
-  if (!c.A) fatal("SetAParcsr: matrix has no device copy (ParcsrSetup not called)",
-                  __FILE__, __LINE__);
+  c.A_src = dev_matrix_of(A);
+  if (!c.A_src) fatal("SetAParcsr: matrix has no device copy (ParcsrSetup not called)",
+                      __FILE__, __LINE__);
+  c.A_view = *c.A_src;
+  if (!c.col16) c.A_view.jo16 = nullptr;
+  c.A = &c.A_view;
   c.host_mv = nullptr;
   return 0;
 }
@@ -305,6 +308,7 @@ int pEVSL_SetAMatvec(pevsl_Data *p, MVFunc func, void *data) {
   p->Amv->func = func;
   p->Amv->data = data;
   c.A = nullptr;
+  c.A_src = nullptr;
   c.host_mv = func;
   c.host_mv_data = data;
   if (p->n > 0) {
@@ -384,6 +388,22 @@ int pEVSL_SetGraphFilter(pevsl_Data *p, int on) {
   if (on != 0 && on != 1) return -1;
   ctx_of(p).graph_filter = on;
   return 0;
+}
+
+int pEVSL_SetColumnOffsets16(pevsl_Data *p, int on) {
+  if (on != 0 && on != 1) return -1;
+  Ctx &c = ctx_of(p);
+  c.col16 = on;
+  if (c.A_src) {
+    c.A_view = *c.A_src;
+    if (!on) c.A_view.jo16 = nullptr;
+  }
+  return 0;
+}
+
+int pEVSL_ColumnOffsets16(pevsl_Data *p) {
+  const Ctx &c = ctx_of(p);
+  return c.A && c.A->jo16 ? 1 : 0;
 }
 
 int pEVSL_SetBatchWave(pevsl_Data *p, int wave) {
@@ -513,6 +533,27 @@ int pEVSL_ParcsrSetup(pevsl_Csr *Ai, pevsl_Parcsr *A) {
   M->csr.nrowblk = (int)rb.size() - 1;
   M->csr.rowblk = M->rowblk.ensure(rb.size());
   PV_HIP(hipMemcpy(M->csr.rowblk, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice));
+  // 16-bit column offsets from each row block's first row, kept when every
+  // nonzero's offset fits (banded orderings such as the natural order of a
+  // grid Laplacian; not random sparsity)
+  std::vector<short> jo((size_t)nnz);
+  bool fits = nnz > 0;
+  for (size_t b = 0; fits && b + 1 < rb.size(); b++) {
+    const int r0 = rb[b];
+    for (int j = Ai->ia[r0]; j < Ai->ia[rb[b + 1]]; j++) {
+      const int d = Ai->ja[j] - r0;
+      if (d < -32768 || d > 32767) {
+        fits = false;
+        break;
+      }
+      jo[(size_t)j] = (short)d;
+    }
+  }
+  if (fits) {
+    M->csr.jo16 = M->jo16.ensure((size_t)nnz);
+    PV_HIP(hipMemcpy(M->csr.jo16, jo.data(), sizeof(short) * (size_t)nnz,
+                     hipMemcpyHostToDevice));
+  }
   M->csr.rowblk_q[0] = M->csr.rowblk;
   M->csr.nrowblk_q[0] = M->csr.nrowblk;
   for (int l = 1; l < 5; l++) {

@@ -66,6 +66,7 @@ struct DevMatrix {
   DBuf<int> rowblk_q[5];  // partitions of the nnz-balanced kernels, Q >= 2
   DBuf<double> a64;
   DBuf<float> a32;
+  DBuf<short> jo16;
 };
 
 // Live per-kernel timing on an instance's stream: HIP events around a run of
@@ -113,7 +114,10 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   rocblas_handle blas = nullptr;
-  const DevCsr *A = nullptr;       // set by SetAParcsr
+  const DevCsr *A = nullptr;       // set by SetAParcsr: &A_view
+  const DevCsr *A_src = nullptr;   // the matrix's device copy
+  DevCsr A_view;                   // A_src, without jo16 when col16 == 0
+  int col16 = 1;                   // 16-bit column offsets where the matrix has them
   MVFunc host_mv = nullptr;        // set by SetAMatvec (host callback)
   void *host_mv_data = nullptr;
   int filter_prec = PEVSL_PREC_FP64;

@@ -148,6 +148,18 @@ __device__ __forceinline__ int xcd_block(int bid, int nb) {
   return x * q + min(x, r) + j;
 }
 
+// Column of nonzero j in a row block that starts at row r0: the 32-bit index,
+// or (J16) r0 plus the nonzero's 16-bit offset from r0 (DevCsr::jo16), which
+// saves 2 of the 4 index bytes per nonzero.
+template <bool J16>
+__device__ __forceinline__ int block_col(const int *__restrict__ ja, const short *__restrict__ jo,
+                                         int r0, int j) {
+  if constexpr (J16)
+    return r0 + (int)jo[j];
+  else
+    return ja[j];
+}
+
 inline int grid_for(size_t work, int threads = kThreads) {
   return (int)((work + threads - 1) / threads);
 }
@@ -220,10 +232,10 @@ k_cheb_vec(int n, const int *__restrict__ ia, const int *__restrict__ ja,
 // row sequentially (the reference's summation order) and applies the
 // recurrence.  A row longer than kStreamNnz forms a block alone and is
 // reduced by the whole block.
-template <typename T>
+template <typename T, bool J16>
 __global__ void __launch_bounds__(kThreads)
 k_cheb_stream(const int *__restrict__ rowblk, const int *__restrict__ ia,
-              const int *__restrict__ ja, const T *__restrict__ a,
+              const int *__restrict__ ja, const short *__restrict__ jo, const T *__restrict__ a,
               const T *__restrict__ vk, const T *__restrict__ vkm1,
               T *__restrict__ vkp1, T *__restrict__ y, T cc, T t, T mu0, T muk,
               int first) {
@@ -235,14 +247,16 @@ k_cheb_stream(const int *__restrict__ rowblk, const int *__restrict__ ia,
   T acc = 0;
   int r = r0 + threadIdx.x;
   if (nz <= kStreamNnz) {
-    for (int q = threadIdx.x; q < nz; q += kThreads) prod[q] = a[j0 + q] * vk[ja[j0 + q]];
+    for (int q = threadIdx.x; q < nz; q += kThreads)
+      prod[q] = a[j0 + q] * vk[block_col<J16>(ja, jo, r0, j0 + q)];
     __syncthreads();
     if (r >= r1) return;
     const int qs = ia[r] - j0, qe = ia[r + 1] - j0;
     for (int q = qs; q < qe; q++) acc += prod[q];
   } else {
     // single long row
-    for (int q = threadIdx.x; q < nz; q += kThreads) acc += a[j0 + q] * vk[ja[j0 + q]];
+    for (int q = threadIdx.x; q < nz; q += kThreads)
+      acc += a[j0 + q] * vk[block_col<J16>(ja, jo, r0, j0 + q)];
     __shared__ T red[kThreads];
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -356,10 +370,10 @@ __device__ __forceinline__ void store_row_nt(T *__restrict__ p, const T (&x)[W])
   }
 }
 
-template <typename T, int B>
+template <typename T, int B, bool J16>
 __global__ void __launch_bounds__(kThreads)
 k_cheb_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
-              const int *__restrict__ ja, const T *__restrict__ a,
+              const int *__restrict__ ja, const short *__restrict__ jo, const T *__restrict__ a,
               const T *__restrict__ vk, const T *__restrict__ vkm1,
               T *__restrict__ vkp1, T *__restrict__ y, T cc, T t,
               const double *__restrict__ mus, int k, int nact) {
@@ -373,7 +387,7 @@ k_cheb_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
   if (staged) {
     for (int q = threadIdx.x; q < nz; q += kThreads) {
       as[q] = a[j0 + q];
-      js[q] = ja[j0 + q];
+      js[q] = block_col<J16>(ja, jo, r0, j0 + q);
     }
     __syncthreads();
   } else if (threadIdx.x != 0) {
@@ -387,7 +401,7 @@ k_cheb_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
   const int qs = ia[r] - j0, qe = ia[r + 1] - j0;
   for (int q = qs; q < qe; q++) {
     const T av = staged ? as[q] : a[j0 + q];
-    const int col = staged ? js[q] : ja[j0 + q];
+    const int col = staged ? js[q] : block_col<J16>(ja, jo, r0, j0 + q);
     T x[B];
     load_row<T, B>(vk + (size_t)col * B, x);
 #pragma unroll
@@ -426,10 +440,10 @@ k_cheb_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
 // filtered vector itself.  Per step it gathers b1 and streams b2, v and out:
 // four vector passes where the three-term form with an accumulator needs
 // five.
-template <typename T, int B>
+template <typename T, int B, bool J16>
 __global__ void __launch_bounds__(kThreads)
 k_clen_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
-              const int *__restrict__ ja, const T *__restrict__ a,
+              const int *__restrict__ ja, const short *__restrict__ jo, const T *__restrict__ a,
               const T *__restrict__ b1, const T *__restrict__ b2,
               const T *__restrict__ v, T *__restrict__ out, T cc, T t,
               const double *__restrict__ mus, int k) {
@@ -443,7 +457,7 @@ k_clen_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
   if (staged) {
     for (int q = threadIdx.x; q < nz; q += kThreads) {
       as[q] = a[j0 + q];
-      js[q] = ja[j0 + q];
+      js[q] = block_col<J16>(ja, jo, r0, j0 + q);
     }
     __syncthreads();
   } else if (threadIdx.x != 0) {
@@ -457,7 +471,7 @@ k_clen_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
   const int qs = ia[r] - j0, qe = ia[r + 1] - j0;
   for (int q = qs; q < qe; q++) {
     const T av = staged ? as[q] : a[j0 + q];
-    const int col = staged ? js[q] : ja[j0 + q];
+    const int col = staged ? js[q] : block_col<J16>(ja, jo, r0, j0 + q);
     T x[B];
     load_row<T, B>(b1 + (size_t)col * B, x);
 #pragma unroll
@@ -1815,9 +1829,14 @@ void cheb_step_stream(const DevCsr &A, const T *vk, const T *vkm1, T *vkp1,
   if (A.n == 0 || A.nrowblk == 0) return;
   const T *a = std::is_same<T, double>::value ? (const T *)A.a64
                                               : (const T *)A.a32;
-  hipLaunchKernelGGL((k_cheb_stream<T>), dim3(A.nrowblk), dim3(kThreads), 0, s,
-                     A.rowblk, A.ia, A.ja, a, vk, vkm1, vkp1, y, (T)cc, (T)t,
-                     (T)mu0, (T)muk, first ? 1 : 0);
+  if (A.jo16)
+    hipLaunchKernelGGL((k_cheb_stream<T, true>), dim3(A.nrowblk), dim3(kThreads), 0, s,
+                       A.rowblk, A.ia, A.ja, A.jo16, a, vk, vkm1, vkp1, y, (T)cc, (T)t,
+                       (T)mu0, (T)muk, first ? 1 : 0);
+  else
+    hipLaunchKernelGGL((k_cheb_stream<T, false>), dim3(A.nrowblk), dim3(kThreads), 0, s,
+                       A.rowblk, A.ia, A.ja, A.jo16, a, vk, vkm1, vkp1, y, (T)cc, (T)t,
+                       (T)mu0, (T)muk, first ? 1 : 0);
 }
 template void cheb_step_stream<double>(const DevCsr &, const double *,
                                        const double *, double *, double *,
@@ -1865,11 +1884,16 @@ void cheb_step_blkrow(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp
   const T *a = std::is_same<T, double>::value ? (const T *)A.a64
                                               : (const T *)A.a32;
   switch (B) {
-#define CASE(BB)                                                                  \
-  case BB:                                                                        \
-    hipLaunchKernelGGL((k_cheb_blkrow<T, BB>), dim3(A.nrowblk), dim3(kThreads), 0, \
-                       s, A.rowblk, A.ia, A.ja, a, vk, vkm1, vkp1, y, (T)cc, (T)t,  \
-                       mus, k, nact);                                             \
+#define CASE(BB)                                                                        \
+  case BB:                                                                              \
+    if (A.jo16)                                                                         \
+      hipLaunchKernelGGL((k_cheb_blkrow<T, BB, true>), dim3(A.nrowblk), dim3(kThreads), 0, \
+                         s, A.rowblk, A.ia, A.ja, A.jo16, a, vk, vkm1, vkp1, y, (T)cc,   \
+                         (T)t, mus, k, nact);                                           \
+    else                                                                                \
+      hipLaunchKernelGGL((k_cheb_blkrow<T, BB, false>), dim3(A.nrowblk), dim3(kThreads), 0, \
+                         s, A.rowblk, A.ia, A.ja, A.jo16, a, vk, vkm1, vkp1, y, (T)cc,   \
+                         (T)t, mus, k, nact);                                           \
     break;
     CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32)
 #undef CASE
@@ -1888,10 +1912,16 @@ void clen_step_blkrow(const DevCsr &A, int B, const T *b1, const T *b2, const T 
   }
   const T *a = std::is_same<T, double>::value ? (const T *)A.a64 : (const T *)A.a32;
   switch (B) {
-#define CASE(BB)                                                                         \
-  case BB:                                                                               \
-    hipLaunchKernelGGL((k_clen_blkrow<T, BB>), dim3(A.nrowblk), dim3(kThreads), 0, s,    \
-                       A.rowblk, A.ia, A.ja, a, b1, b2, v, out, (T)cc, (T)t, mus, k);    \
+#define CASE(BB)                                                                        \
+  case BB:                                                                              \
+    if (A.jo16)                                                                         \
+      hipLaunchKernelGGL((k_clen_blkrow<T, BB, true>), dim3(A.nrowblk), dim3(kThreads), 0, \
+                         s, A.rowblk, A.ia, A.ja, A.jo16, a, b1, b2, v, out, (T)cc, (T)t, \
+                         mus, k);                                                       \
+    else                                                                                \
+      hipLaunchKernelGGL((k_clen_blkrow<T, BB, false>), dim3(A.nrowblk), dim3(kThreads), 0, \
+                         s, A.rowblk, A.ia, A.ja, A.jo16, a, b1, b2, v, out, (T)cc, (T)t, \
+                         mus, k);                                                       \
     break;
     CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32)
 #undef CASE

@@ -27,7 +27,14 @@ struct DevCsr {
   // kStreamNnz/Q nonzeros per block (rowblk_q[0] == rowblk)
   int *rowblk_q[5] = {};
   int nrowblk_q[5] = {};
+  // ja as 16-bit offsets from the first row of each nonzero's block in
+  // `rowblk` (null when an offset does not fit in a short): the CSR-stream
+  // and row-per-thread block kernels then read 2 index bytes per nonzero
+  short *jo16 = nullptr;
 };
+
+// index bytes per nonzero read by the CSR-stream and row-block kernels
+inline int index_bytes(const DevCsr &A) { return A.jo16 ? 2 : 4; }
 
 constexpr int kStreamRows = 256;
 constexpr int kStreamNnz = 2048;

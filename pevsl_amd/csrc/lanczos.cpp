@@ -70,10 +70,11 @@ struct Slice {
   std::vector<double> vecs;  // accepted Ritz vectors (host, n x nev)
 };
 
-// Clenshaw step: gathered b1 (counted once), b2 (if any), v, out
-size_t clen_launch_bytes(const DevCsr &A, int elem, int ncol, bool has_b2) {
+// Clenshaw step: gathered b1 (counted once), b2 (if any), v, out; idx =
+// index bytes per nonzero the kernel reads (2 with 16-bit column offsets)
+size_t clen_launch_bytes(const DevCsr &A, int elem, int ncol, bool has_b2, int idx) {
   const double vec = (double)A.n * elem;
-  return (size_t)((double)A.nnz * (elem + 4) + 4.0 * (A.n + 1) +
+  return (size_t)((double)A.nnz * (elem + idx) + 4.0 * (A.n + 1) +
                   (has_b2 ? 4.0 : 3.0) * vec * ncol);
 }
 
@@ -89,9 +90,9 @@ double clen_seg_launch_bytes(const DevCsr &A, int elem, int seg, int d, int nseg
   return b;
 }
 
-size_t filter_launch_bytes(const DevCsr &A, int elem, int ncol, bool first) {
+size_t filter_launch_bytes(const DevCsr &A, int elem, int ncol, bool first, int idx) {
   const double vec = (double)A.n * elem;
-  return (size_t)((double)A.nnz * (elem + 4) + 4.0 * (A.n + 1) +
+  return (size_t)((double)A.nnz * (elem + idx) + 4.0 * (A.n + 1) +
                   (first ? 3.0 : 5.0) * vec * ncol);
 }
 
@@ -149,7 +150,7 @@ static void cheb_filter_f32(Ctx &c, const pevsl_Polparams *pol, const float *x32
                            mu[k], k == 1, lanes, s);
     vkm1 = vk;
     vk = vkp1;
-    bytes += (double)filter_launch_bytes(A, 4, 1, k == 1);
+    bytes += (double)filter_launch_bytes(A, 4, 1, k == 1, stream ? index_bytes(A) : 4);
   }
   if (tok >= 0) c.kt.end(s, tok, m, bytes);
   c.kstats.filter_bytes += bytes;
@@ -213,7 +214,8 @@ void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
                               lanes, s);
       vkm1 = vk;
       vk = vkp1;
-      c.kstats.filter_bytes += (double)filter_launch_bytes(A, 8, 1, k == 1);
+      c.kstats.filter_bytes +=
+          (double)filter_launch_bytes(A, 8, 1, k == 1, stream ? index_bytes(A) : 4);
     }
     if (tok >= 0) c.kt.end(s, tok, m, c.kstats.filter_bytes - bytes0);
   } else {
@@ -670,7 +672,8 @@ void BatchSolver::enqueue_block(const std::vector<int> &grp, int B, int kind,
         int nact = 0;  // columns whose series has reached degree d + 1
         while (nact < g && degs[(size_t)nact] >= d + 1) nact++;
         if (b1) {
-          const double lb = (double)clen_launch_bytes(A, (int)elem, nact, b2 != nullptr);
+          const double lb = (double)clen_launch_bytes(A, (int)elem, nact, b2 != nullptr,
+                                                      bal ? 4 : index_bytes(A));
           kbytes += lb;
           c_.kstats.filter_bytes += lb;
           c_.kstats.filter_steps += nact;
@@ -707,7 +710,8 @@ void BatchSolver::enqueue_block(const std::vector<int> &grp, int B, int kind,
       else
         cheb_step_blk<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd,
                          musp, d, nact, c_.stream);
-      const double lb = (double)filter_launch_bytes(A, (int)elem, nact, d == 1);
+      const double lb =
+          (double)filter_launch_bytes(A, (int)elem, nact, d == 1, rowk && !bal ? index_bytes(A) : 4);
       kbytes += lb;
       c_.kstats.filter_bytes += lb;
       c_.kstats.filter_steps += nact;

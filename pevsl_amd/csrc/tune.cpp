@@ -15,8 +15,10 @@ using namespace pevsl;
 
 namespace {
 
-double step_bytes(const DevCsr &A, int elem, int ncol) {
-  return (double)A.nnz * (elem + 4) + 4.0 * (A.n + 1) + 5.0 * (double)A.n * elem * ncol;
+// idx: index bytes per nonzero the kernel reads (2 for the row-block kernels
+// on a matrix with 16-bit column offsets)
+double step_bytes(const DevCsr &A, int elem, int ncol, int idx = 4) {
+  return (double)A.nnz * (elem + idx) + 4.0 * (A.n + 1) + 5.0 * (double)A.n * elem * ncol;
 }
 
 // lanes == 0: the CSR-stream kernel
@@ -170,7 +172,8 @@ int pEVSL_Autotune(pevsl_Data *p, int reps, pevsl_TuneResult *res) {
       if (f32) {
         const double tc = time_blk<float>(c, B, reps, mus, 2);
         res->t_blockclen[i] = tc;
-        res->bytes_clen[i] = step_bytes(*c.A, elem, B) - (double)c.A->n * B * elem;
+        res->bytes_clen[i] =
+            step_bytes(*c.A, elem, B, index_bytes(*c.A)) - (double)c.A->n * B * elem;
         if (tc / B < bestc) {
           bestc = tc / B;
           bw = B;
@@ -211,9 +214,10 @@ int pEVSL_BenchBlock(pevsl_Data *p, int B, int reps, double *sec, double *bytes)
   std::vector<double> mh(3 * 64, 0.01);
   PV_HIP(hipMemcpy(mus.p, mh.data(), sizeof(double) * mh.size(), hipMemcpyHostToDevice));
   *sec = f32 ? time_blk<float>(c, B, reps, mus, kind) : time_blk<double>(c, B, reps, mus, kind);
+  const int idx = (kind == 1 || kind == 2) ? index_bytes(*c.A) : 4;
   *bytes = (kind == 2 || kind == 4 || kind == 5)
-               ? step_bytes(*c.A, elem, B) - (double)c.A->n * B * elem
-                                    : step_bytes(*c.A, elem, B);
+               ? step_bytes(*c.A, elem, B, idx) - (double)c.A->n * B * elem
+               : step_bytes(*c.A, elem, B, idx);
   return 0;
 }
 

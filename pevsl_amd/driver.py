@@ -237,7 +237,7 @@ class Instance:
     def __init__(self, lib=None, matrix=None, prec=capi.PEVSL_PREC_FP64,
                  rand_mode=capi.PEVSL_RAND_LIBC, seed=0, keep_vectors=False,
                  lanes=0, block_width=0, ritz_refine=-1, mixed_tol=None,
-                 block_rowkernel=-1, basis_prec=capi.PEVSL_PREC_FP64):
+                 block_rowkernel=-1, basis_prec=capi.PEVSL_PREC_FP64, col16=True):
         self.lib = lib or capi.load()
         self.p = capi.DataP()
         rc = self.lib.pEVSL_Start(capi.MPI_COMM_WORLD, C.byref(self.p))
@@ -252,6 +252,8 @@ class Instance:
         self.lib.pEVSL_SetRitzRefine(self.p, ritz_refine)
         self.lib.pEVSL_SetBlockKernel(self.p, block_rowkernel)
         self.lib.pEVSL_SetBasisPrecision(self.p, basis_prec)
+        if not col16:
+            self.lib.pEVSL_SetColumnOffsets16(self.p, 0)
         if mixed_tol is not None:
             self.lib.pEVSL_SetMixedTol(self.p, mixed_tol)
 
