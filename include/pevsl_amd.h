@@ -210,16 +210,19 @@ int pEVSL_CGS_DGKS(pevsl_Data *pevsl, int k, int i_max, pevsl_Parvecs *Q, pevsl_
                    double *nrmv, double *w);
 
 /* Gram products G = U^T U and H = U^T A U of the FP64 Rayleigh-Ritz stage
- * (mixed path): 0 = rocBLAS strided-batched DGEMM over row parts,
- * 1 = the library's own FP64 MFMA kernel (v_mfma_f64_16x16x4 tiles over row
- * parts, summed in a fixed order), -1 (default) = the MFMA kernel below
- * 65,536 rows, where rocBLAS would run one unsplit DGEMM, rocBLAS above. */
+ * (mixed path): 0 = rocBLAS strided-batched DGEMM over row parts (whole
+ * products), 1 = the library's own FP64 MFMA kernel (v_mfma_f64_16x16x4
+ * tiles over row parts, summed in a fixed order; only the tiles on and above
+ * the diagonal, the triangle the pencil's solver reads), -1 (default) = 1. */
 int pEVSL_SetGramKernel(pevsl_Data *pevsl, int kernel);
 
 /* Times both Gram kernels on C = X^T Y for random n x m FP64 X, Y (HIP
  * events, `reps` calls each) and returns the largest difference of the two
- * results relative to the largest entry. */
-int pEVSL_BenchGram(pevsl_Data *pevsl, int n, int m, int reps, double *sec_blas,
+ * results relative to the largest entry.  upper = 1: the MFMA kernel forms
+ * only the upper triangle (as for the Rayleigh-Ritz pencil) and returns it
+ * in both triangles; the difference is taken over the upper triangle, and
+ * is HUGE_VAL if the lower triangle does not mirror it exactly. */
+int pEVSL_BenchGram(pevsl_Data *pevsl, int n, int m, int reps, int upper, double *sec_blas,
                     double *sec_mfma, double *maxdiff);
 
 /* Diagnostics of slice s of the last ChebLanNr / ChebLanNrBatch call. */

@@ -316,7 +316,7 @@ SIGNATURES = {
     "pEVSL_SetGramKernel": (C.c_int, [DataP, C.c_int]),
     "pEVSL_SetColumnOffsets16": (C.c_int, [DataP, C.c_int]),
     "pEVSL_ColumnOffsets16": (C.c_int, [DataP]),
-    "pEVSL_BenchGram": (C.c_int, [DataP, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+    "pEVSL_BenchGram": (C.c_int, [DataP, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pEVSL_CGS_DGKS": (C.c_int, [DataP, C.c_int, C.c_int, C.POINTER(Parvecs), C.POINTER(Parvec),
                                  C.POINTER(C.c_double), C.c_void_p]),

@@ -1100,14 +1100,26 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 template <int WB, int KR>
 __global__ void __launch_bounds__(kThreads)
 k_gram_mfma(int n, int m1, int m2, const double *__restrict__ X, size_t ldx,
-            const double *__restrict__ Y, size_t ldy, int chunk, double *__restrict__ part) {
+            const double *__restrict__ Y, size_t ldy, int chunk, int upper,
+            double *__restrict__ part) {
   constexpr int T = 32 * WB;
   constexpr int PER = KR * T / kThreads;  // staged elements per thread and matrix
   __shared__ double Xs[KR][T + 1];
   __shared__ double Ys[KR][T + 1];
-  const int tiles1 = (m1 + T - 1) / T;
-  const int i0 = (int)(blockIdx.x % tiles1) * T;
-  const int j0 = (int)(blockIdx.x / tiles1) * T;
+  int ti, tj;
+  if (upper) {
+    // tiles on and above the diagonal, column by column: x = tj(tj+1)/2 + ti
+    const int x = (int)blockIdx.x;
+    tj = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
+    while ((tj + 1) * (tj + 2) / 2 <= x) tj++;
+    while (tj * (tj + 1) / 2 > x) tj--;
+    ti = x - tj * (tj + 1) / 2;
+  } else {
+    const int tiles1 = (m1 + T - 1) / T;
+    ti = (int)(blockIdx.x % tiles1);
+    tj = (int)(blockIdx.x / tiles1);
+  }
+  const int i0 = ti * T, j0 = tj * T;
   const int r0 = (int)blockIdx.y * chunk;
   const int r1 = min(n, r0 + chunk);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -2079,15 +2091,38 @@ int dot_partials(int n, const double *x, const double *y, double *part,
 }
 
 void gram_tn_mfma(int n, int m1, const double *X, size_t ldx, int m2, const double *Y,
-                  size_t ldy, int chunk, int nparts, double *part, hipStream_t s) {
+                  size_t ldy, int chunk, int nparts, bool upper, double *part, hipStream_t s) {
   if (n <= 0 || m1 <= 0 || m2 <= 0 || nparts <= 0) return;
+  if (upper && m1 != m2) return;
   // 64 x 64 workgroup tiles, 64-row steps: measured on n = 1e6 rows, m = 800
   // (FP64), 29.0 ms against 30.0 / 31.0 ms for 32- / 16-row steps and
   // 33.3 ms for 128 x 128 tiles
   constexpr int T = 64;
-  const int tiles = ((m1 + T - 1) / T) * ((m2 + T - 1) / T);
+  const int t1 = (m1 + T - 1) / T, t2 = (m2 + T - 1) / T;
+  const int tiles = upper ? t1 * (t1 + 1) / 2 : t1 * t2;
   hipLaunchKernelGGL((k_gram_mfma<2, 64>), dim3(tiles, nparts), dim3(kThreads), 0, s, n, m1, m2,
-                     X, ldx, Y, ldy, chunk, part);
+                     X, ldx, Y, ldy, chunk, upper ? 1 : 0, part);
+}
+
+// C = sum of the upper triangles of the m x m parts, in part order (as
+// k_sum_parts sums each element), written to both triangles of C
+__global__ void k_sum_parts_upper(int nparts, int m, const double *__restrict__ part,
+                                  double *__restrict__ C) {
+  const size_t e = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  const size_t nel = (size_t)m * m;
+  if (e >= nel) return;
+  const int r = (int)(e % m), c = (int)(e / m);
+  if (r > c) return;
+  double v = 0.0;
+  for (int p = 0; p < nparts; p++) v += part[(size_t)p * nel + e];
+  C[e] = v;
+  C[(size_t)r * m + c] = v;
+}
+
+void sum_parts_upper(const double *part, int nparts, int m, double *C, hipStream_t s) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_sum_parts_upper, dim3(grid_for((size_t)m * m)), dim3(kThreads), 0, s,
+                     nparts, m, part, C);
 }
 
 void scale_vec(int n, double *x, const double *inv_scale_dev, double hostscale,

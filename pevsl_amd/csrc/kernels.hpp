@@ -143,8 +143,12 @@ void copy_f64(size_t n, const double *x, double *y, hipStream_t s);
 // part[p] (m1 x m2, column-major) = X(rows p*chunk .. , :)^T Y(same rows, :)
 // for p < nparts (the last part ends at row n), FP64 MFMA (hand-written
 // v_mfma_f64_16x16x4 tiles); X and Y column-major with leading dims ldx, ldy.
+// upper (m1 == m2): only the 64 x 64 tiles on and above the diagonal.
 void gram_tn_mfma(int n, int m1, const double *X, size_t ldx, int m2, const double *Y,
-                  size_t ldy, int chunk, int nparts, double *part, hipStream_t s);
+                  size_t ldy, int chunk, int nparts, bool upper, double *part, hipStream_t s);
+// C (m x m) = sum over p < nparts of the upper triangles of part[p], in part
+// order, stored in both triangles (a symmetric C)
+void sum_parts_upper(const double *part, int nparts, int m, double *C, hipStream_t s);
 // out[e] = sum_p part[p*nel + e] over p = 0..nparts-1 in order
 void sum_parts(const double *part, int nparts, size_t nel, double *out, hipStream_t s);
 // part[blk] = sum x_i*y_i over the block's rows; returns nblk

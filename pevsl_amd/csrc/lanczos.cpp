@@ -246,22 +246,31 @@ namespace {
 // chunks computed as one strided-batched DGEMM (MFMA), whose partial
 // products are then summed in a fixed order.
 constexpr int kGramParts = 64;
+// upper (m1 == m2): the caller reads only the upper triangle of C (the
+// Rayleigh-Ritz pencil, solved with rocblas_fill_upper).  The MFMA kernel then
+// forms only the tiles on and above the diagonal and returns C symmetric.
 void gram_tn(rocblas_handle h, hipStream_t st, int n, int m1, const double *A, int lda, int m2,
-             const double *B, int ldb, double *C, DBuf<double> &work, int kernel) {
+             const double *B, int ldb, double *C, DBuf<double> &work, int kernel,
+             bool upper = false) {
   const double one = 1.0, zero = 0.0;
-  // default (-1): the library's MFMA kernel where the row split below would
-  // fall back to one unsplit DGEMM (n < 64 x 1024: 6.4 ms against 0.11 ms at
-  // n = 40,000, m = 130), rocBLAS above (at n = 1e6, m = 800: 23.1 ms
-  // against 29.0 ms)
-  if (kernel < 0) kernel = n < kGramParts * 1024 ? 1 : 0;
+  // default (-1): the library's MFMA kernel for the upper triangle, and for
+  // the full product where the row split below would fall back to one
+  // unsplit DGEMM (n < 64 x 1024: 6.4 ms against 0.11 ms at n = 40,000,
+  // m = 130); rocBLAS for the full product above (at n = 1e6, m = 800:
+  // 23.1 ms against 29.0 ms)
+  upper = upper && m1 == m2;
+  if (kernel < 0) kernel = (upper || n < kGramParts * 1024) ? 1 : 0;
   if (kernel == 1) {
     // hand-written FP64 MFMA tiles over at most kGramParts row parts of >= 1024 rows
     const int chunk = std::max(1024, (n + kGramParts - 1) / kGramParts);
     const int parts = (n + chunk - 1) / chunk;
     const size_t blk = (size_t)m1 * m2;
     double *wk = work.ensure(blk * parts);
-    gram_tn_mfma(n, m1, A, (size_t)lda, m2, B, (size_t)ldb, chunk, parts, wk, st);
-    sum_parts(wk, parts, blk, C, st);
+    gram_tn_mfma(n, m1, A, (size_t)lda, m2, B, (size_t)ldb, chunk, parts, upper, wk, st);
+    if (upper)
+      sum_parts_upper(wk, parts, m1, C, st);
+    else
+      sum_parts(wk, parts, blk, C, st);
     return;
   }
   const int chunk = n / kGramParts;
@@ -1124,8 +1133,10 @@ void BatchSolver::ritz_rr(int s, double tol) {
   ework.ensure((size_t)nsel);
   {
     DBuf<double> gw;
-    gram_tn(c_.blas, st, n_, nsel, U.p, (int)ld, nsel, U.p, (int)ld, G.p, gw, c_.gram_kernel);
-    gram_tn(c_.blas, st, n_, nsel, U.p, (int)ld, nsel, W.p, (int)ld, H.p, gw, c_.gram_kernel);
+    gram_tn(c_.blas, st, n_, nsel, U.p, (int)ld, nsel, U.p, (int)ld, G.p, gw, c_.gram_kernel,
+            true);
+    gram_tn(c_.blas, st, n_, nsel, U.p, (int)ld, nsel, W.p, (int)ld, H.p, gw, c_.gram_kernel,
+            true);
   }
   PV_BLAS(rocsolver_dsygvd(c_.blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper,
                            nsel, H.p, nsel, G.p, nsel, th.p, ework.p, info.p));
@@ -1708,10 +1719,11 @@ int pEVSL_SetGramKernel(pevsl_Data *pevsl, int kernel) {
   return 0;
 }
 
-int pEVSL_BenchGram(pevsl_Data *pevsl, int n, int m, int reps, double *sec_blas,
+int pEVSL_BenchGram(pevsl_Data *pevsl, int n, int m, int reps, int upper, double *sec_blas,
                     double *sec_mfma, double *maxdiff) {
   Ctx &c = ctx_of(pevsl);
   if (n < 1 || m < 1 || reps < 1 || !sec_blas || !sec_mfma || !maxdiff) return -1;
+  const bool up = upper != 0;
   const size_t ld = ((size_t)n + 63) / 64 * 64;
   DBuf<double> X, Y, C0, C1, work;
   X.ensure(ld * m);
@@ -1722,12 +1734,15 @@ int pEVSL_BenchGram(pevsl_Data *pevsl, int n, int m, int reps, double *sec_blas,
   C1.ensure((size_t)m * m);
   double *out[2] = {C0.p, C1.p};
   double *sec[2] = {sec_blas, sec_mfma};
+  // rocBLAS forms the whole product; the MFMA kernel the whole product or
+  // (upper) the upper triangle, returned symmetric
   for (int kern = 0; kern < 2; kern++) {
-    gram_tn(c.blas, c.stream, n, m, X.p, (int)ld, m, Y.p, (int)ld, out[kern], work, kern);
+    const bool u = up && kern == 1;
+    gram_tn(c.blas, c.stream, n, m, X.p, (int)ld, m, Y.p, (int)ld, out[kern], work, kern, u);
     EvTimer ev;
     ev.start(c.stream);
     for (int r = 0; r < reps; r++)
-      gram_tn(c.blas, c.stream, n, m, X.p, (int)ld, m, Y.p, (int)ld, out[kern], work, kern);
+      gram_tn(c.blas, c.stream, n, m, X.p, (int)ld, m, Y.p, (int)ld, out[kern], work, kern, u);
     ev.stop(c.stream);
     *sec[kern] = ev.seconds() / reps;
   }
@@ -1735,10 +1750,17 @@ int pEVSL_BenchGram(pevsl_Data *pevsl, int n, int m, int reps, double *sec_blas,
   PV_HIP(hipMemcpy(h0.data(), C0.p, sizeof(double) * h0.size(), hipMemcpyDeviceToHost));
   PV_HIP(hipMemcpy(h1.data(), C1.p, sizeof(double) * h1.size(), hipMemcpyDeviceToHost));
   double dmax = 0.0, cmax = 0.0;
-  for (size_t i = 0; i < h0.size(); i++) {
-    dmax = std::max(dmax, std::fabs(h0[i] - h1[i]));
-    cmax = std::max(cmax, std::fabs(h0[i]));
-  }
+  for (int cc = 0; cc < m; cc++)
+    for (int r = 0; r < m; r++) {
+      const size_t i = (size_t)cc * m + r;
+      if (up && r > cc) {
+        // the lower triangle mirrors the upper one exactly
+        if (h1[i] != h1[(size_t)r * m + cc]) dmax = HUGE_VAL;
+        continue;
+      }
+      dmax = std::max(dmax, std::fabs(h0[i] - h1[i]));
+      cmax = std::max(cmax, std::fabs(h0[i]));
+    }
   *maxdiff = cmax > 0 ? dmax / cmax : dmax;
   return 0;
 }

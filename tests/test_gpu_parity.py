@@ -328,26 +328,31 @@ def test_cgs_dgks_against_oracle(lib, n, k, i_max, near):
         assert np.max(np.abs(Q.T @ vg)) <= 1e-13 * scale
 
 
+@pytest.mark.parametrize("upper", [0, 1])
 @pytest.mark.parametrize("n,m", [(1000, 7), (40000, 130), (100003, 65), (250000, 100),
                                  (1000000, 800)])
-def test_gram_mfma_against_rocblas(lib, n, m):
+def test_gram_mfma_against_rocblas(lib, n, m, upper):
     """The library's FP64 MFMA Gram kernel (v_mfma_f64_16x16x4 tiles, row
     parts summed in a fixed order) against the rocBLAS strided-batched DGEMM
     path on C = X^T Y for random X, Y: partial 64 x 64 tiles (m = 7, 65, 100,
     130), row counts that leave short parts and partial 64-row steps.  The
-    two summation orders differ in rounding only."""
-    banner("Gram X^T Y, MFMA kernel vs rocBLAS, n=%d m=%d" % (n, m))
+    two summation orders differ in rounding only.  upper: the kernel forms
+    only the tiles on and above the diagonal (1, 3, 3, 6, 91 of 1, 9, 4, 4,
+    169 here) and returns the upper triangle mirrored; the lower triangle
+    must mirror it bit for bit."""
+    banner("Gram X^T Y, MFMA kernel vs rocBLAS, n=%d m=%d upper=%d" % (n, m, upper))
     inst = driver.Instance(lib)
     try:
         lib.pEVSL_SetProbSizes(inst.p, n, n, 0)
         tb, tm, diff = C.c_double(), C.c_double(), C.c_double()
-        assert lib.pEVSL_BenchGram(inst.p, n, m, 3, C.byref(tb), C.byref(tm), C.byref(diff)) == 0
+        assert lib.pEVSL_BenchGram(inst.p, n, m, 3, upper, C.byref(tb), C.byref(tm),
+                                   C.byref(diff)) == 0
     finally:
         inst.close()
     flops = 2.0 * n * m * m
-    print("rocBLAS %.3f ms (%.1f TFLOP/s), MFMA kernel %.3f ms (%.1f TFLOP/s), max diff %.2e" % (
-        tb.value * 1e3, flops / tb.value / 1e12, tm.value * 1e3, flops / tm.value / 1e12,
-        diff.value), flush=True)
+    print("rocBLAS %.3f ms (%.1f TFLOP/s), MFMA kernel %.3f ms (%.1f TFLOP/s of the whole "
+          "product), max diff %.2e" % (tb.value * 1e3, flops / tb.value / 1e12, tm.value * 1e3,
+                                       flops / tm.value / 1e12, diff.value), flush=True)
     assert diff.value < 1e-12
 
 
