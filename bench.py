@@ -128,7 +128,18 @@ def slice_checks(g, sli, s, o):
             "tol_accept": o["info"]["tol_accept"] if o["info"] else None}
 
 
-def roofline_of(ktimes, traffic, prefer=None):
+# kernels compiled in two forms, 16-bit column offsets (true) or 32-bit
+# indices (false): the profiler names carry the flag, the timers do not
+J16_KERNELS = ("k_clen_blkrow<", "k_cheb_blkrow<", "k_cheb_stream<")
+
+
+def profiler_name(name, col16):
+    if name.startswith(J16_KERNELS):
+        return name[:-1] + (",true>" if col16 else ",false>")
+    return name
+
+
+def roofline_of(ktimes, traffic, prefer=None, col16=False):
     """Roofline object of the kernel with the most time among the filter
     kernels (names k_*) and Gram-Schmidt passes."""
     if not ktimes:
@@ -140,7 +151,7 @@ def roofline_of(ktimes, traffic, prefer=None):
     ach = byt / per / 1e9 if per > 0 else 0.0
     return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": traffic.get(name.replace(" ", "")),
+            "traffic": traffic.get(profiler_name(name, col16).replace(" ", "")),
             "bytes_per_launch": byt, "us_per_launch": round(per * 1e6, 3),
             "launches": t["launches"], "seconds": round(t["seconds"], 4)}
 
@@ -495,7 +506,7 @@ def main():
                                         "GBs": round(v["bytes"] / max(v["seconds"], 1e-12) / 1e9, 1)}
                                    for k_, v in kt.items()}}
         filt = {k_: v for k_, v in kt.items() if k_.startswith("k_")}
-        roof = roofline_of(filt, traffic)
+        roof = roofline_of(filt, traffic, col16=out["config"]["col16"])
         if roof and out["config"]["col16"] and roof["kernel"].startswith(
                 ("k_clen_blkrow", "k_cheb_blkrow", "k_cheb_stream")):
             # the kernel read 2-byte column offsets: `achieved` counts them;
@@ -558,6 +569,7 @@ def main():
                 log("fp64 slice %d deg %d: %d/%d in %.1f s (its %d, mixed its %d)" % (
                     s, degs[s], r["found"], r["exact"], dt, r["its"], its[s]))
             kt64 = inst64.kernel_times()
+            c16_64 = bool(lib.pEVSL_ColumnOffsets16(inst64.p))
             inst64.close()
             proj, coef = fp64_model(meas, [(degs[s], its[s]) for s in range(ns)])
             proj += out["slicing"]["dos_slicing_s"] + out["slicing"]["findpol_s"]
@@ -566,7 +578,7 @@ def main():
                   "projection": "sum over the 20 slices of cf*deg*(its+1) + cr*its^2/2 + co*its "
                                 "with the job's step counts, plus the job's DOS/slicing/FindPol time",
                   "roofline": roofline_of({k_: v for k_, v in kt64.items() if k_.startswith("k_")},
-                                          traffic)}
+                                          traffic, col16=c16_64)}
             if out.get("job"):
                 fp["projected_eigenpairs_per_s"] = round(out["job"]["found"] / proj, 3)
                 fp["mixed_speedup_vs_projected_fp64"] = round(proj / out["job"]["seconds"], 3)
