@@ -15,7 +15,7 @@ The GPU tests (tests/test_gpu_powerlaw.py) solve the same slices and compare
 counts and eigenvalues.  Run from the repository root:
 
     python tests/golden/make_powerlaw_golden.py 200000 8 0,1,2,3,4,5,6,7   # ~5 min, 8 threads
-    python tests/golden/make_powerlaw_golden.py 2000000 16 7,8            # ~70 min, 8 threads
+    python tests/golden/make_powerlaw_golden.py 2000000 16 7,8            # ~2 h per slice, 8 threads
 """
 import json
 import os
@@ -49,6 +49,7 @@ def main():
     out = {"recipe": "driver.powerlaw_sym(%d, seed=%d)" % (N, SEED), "n": N,
            "nnz": int(ia[-1]), "lmin": lmin, "lmax": lmax, "bounds_its": its,
            "interval": [edges[0], edges[-1]], "nslices": NSLICES, "slices": []}
+    path = os.path.join(ROOT, "tests", "golden", "powerlaw_n%d.json" % N)
     for s in SOLVE:
         intv = [float(edges[s]), float(edges[s + 1]), lmin, lmax]
         pol = orc.Pol(intv)
@@ -63,10 +64,12 @@ def main():
         out["slices"].append({"slice": s, "intv": intv[:2], "deg": pol.deg, "its": r["its"],
                               "nbreak": r["nbreak"], "lam": lam.tolist(),
                               "max_res": float(np.max(r["res"])) if len(lam) else None})
-    path = os.path.join(ROOT, "tests", "golden", "powerlaw_n%d.json" % N)
-    with open(path, "w") as f:
-        json.dump(out, f, indent=1)
-    print("wrote %s in %.0f s" % (path, time.time() - t0), flush=True)
+        # written after every slice: a long run keeps the slices it finished
+        with open(path + ".tmp", "w") as f:
+            json.dump(out, f, indent=1)
+        os.replace(path + ".tmp", path)
+        print("wrote %s (%d slices) in %.0f s" % (path, len(out["slices"]), time.time() - t0),
+              flush=True)
 
 
 if __name__ == "__main__":
