@@ -247,7 +247,13 @@ def lap_potential(nx, ny, nz, w, seed):
     return ia, ja, a, (0.0, top)
 
 
-def draw_matrix(rng):
+def draw_matrix(rng, grid=None):
+    """Training draw: 3-D sides 12-64 (80%) or 2-D 40-400; grid = (lo, hi):
+    3-D sides drawn in [lo, hi] only (the evaluation range)."""
+    if grid is not None:
+        nx, ny, nz = (int(v) for v in rng.integers(grid[0], grid[1] + 1, 3))
+        w = 0.0 if rng.random() < 0.35 else float(rng.uniform(0.0, 2.0))
+        return nx, ny, nz, w
     if rng.random() < 0.8:
         nx, ny, nz = (int(v) for v in rng.integers(12, 65, 3))
     else:
@@ -277,7 +283,7 @@ def draw_interval_irregular(rng, lmin, lmax):
 
 
 def gen_moments(lib, nmat, seed, log=print, full=(300, 64), cheap=None, family="lap",
-                nmin=100000, nmax=500000):
+                nmin=100000, nmax=500000, grid=None):
     """Per matrix: full and cheap KPM moments from this library's Kpmdos.
     family "lap": Laplacians with potentials framed by Gershgorin bounds;
     "powerlaw": driver.powerlaw_sym matrices with n log-uniform in
@@ -292,7 +298,7 @@ def gen_moments(lib, nmat, seed, log=print, full=(300, 64), cheap=None, family="
             ia, ja, a = driver.powerlaw_sym(n, seed=seed * 100003 + j)
             nx, ny, nz, w = n, 1, 1, -1.0
         else:
-            nx, ny, nz, w = draw_matrix(rng)
+            nx, ny, nz, w = draw_matrix(rng, grid)
             ia, ja, a, (lmin, lmax) = lap_potential(nx, ny, nz, w, seed * 100003 + j)
         M = driver.Matrix(lib, ia, ja, a)
         inst = driver.Instance(lib, M)
@@ -311,7 +317,7 @@ def gen_moments(lib, nmat, seed, log=print, full=(300, 64), cheap=None, family="
             M.free()
         mats.append({"dims": (nx, ny, nz), "w": w, "frame": (lmin, lmax), "n": nx * ny * nz,
                      "mu_full": mu_full, "mu_cheap": mu_cheap})
-        if (j + 1) % (10 if family == "powerlaw" else 100) == 0 or j + 1 == nmat:
+        if (j + 1) % (10 if family == "powerlaw" or grid else 100) == 0 or j + 1 == nmat:
             log("moments: %d/%d matrices, %.1f s" % (j + 1, nmat, time.time() - t0))
             sys.stdout.flush()
     return mats
@@ -394,6 +400,37 @@ def evaluate(lib, mlp, X, Q, T, meta, nslices=10):
     return out
 
 
+def eval_only(lib, args):
+    """Slice quality of saved weights on matrices drawn outside (or inside)
+    the training range, judged by full KPM, against the cheap KPM alone."""
+    grid = None
+    if args.family == "lap" and (args.grid_min or args.grid_max):
+        grid = (args.grid_min or 12, args.grid_max or args.grid_min)
+    t0 = time.time()
+    mats = gen_moments(lib, args.nmat, args.seed, family=args.family, nmin=args.nmin,
+                       nmax=args.nmax, grid=grid)
+    X, Q, T, meta = make_pairs(lib, mats, args.nint, args.seed)
+    mlp = Mlp.load(args.eval, lib)
+    try:
+        slices = (10, 16) if args.family == "powerlaw" else (10, 20)
+        rec = {"weights": os.path.basename(args.eval), "family": args.family,
+               "matrices": len(mats), "samples": int(len(X)), "seed": args.seed,
+               "sizes": {"n_min": int(min(m["n"] for m in mats)),
+                         "n_max": int(max(m["n"] for m in mats))},
+               "seconds": round(time.time() - t0, 1)}
+        if grid:
+            rec["grid"] = list(grid)
+        for ns in slices:
+            rec["eval_%d_slices" % ns] = evaluate(lib, mlp, X, Q, T, meta, ns)
+    finally:
+        mlp.free()
+    if args.eval_out:
+        with open(args.eval_out, "w") as f:
+            json.dump(rec, f, indent=1)
+    print(json.dumps(rec))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
@@ -417,6 +454,13 @@ def main(argv=None):
                     help="held-out matrices: the last --holdout share, or every 10th")
     ap.add_argument("--nmin", type=int, default=100000)
     ap.add_argument("--nmax", type=int, default=500000)
+    ap.add_argument("--eval", default=None, metavar="WEIGHTS",
+                    help="no training: evaluate these weights on freshly drawn matrices "
+                         "(--family, --nmat, --nint, and --grid-min/--grid-max or --nmin/--nmax)")
+    ap.add_argument("--grid-min", type=int, default=None,
+                    help="with --eval, family lap: 3-D grid sides drawn in [grid-min, grid-max]")
+    ap.add_argument("--grid-max", type=int, default=None)
+    ap.add_argument("--eval-out", default=None, help="with --eval: write the record here")
     args = ap.parse_args(argv)
     for name in ("nmat", "nint", "epochs", "batch"):
         if getattr(args, name) < 1:
@@ -427,8 +471,15 @@ def main(argv=None):
         ap.error("--holdout must lie in (0, 1)")
     if not 1 <= args.nmin <= args.nmax:
         ap.error("--nmin and --nmax must satisfy 1 <= nmin <= nmax")
+    if args.grid_min is not None or args.grid_max is not None:
+        lo = args.grid_min if args.grid_min is not None else 12
+        hi = args.grid_max if args.grid_max is not None else lo
+        if not 2 <= lo <= hi:
+            ap.error("--grid-min and --grid-max must satisfy 2 <= min <= max")
     lib = capi.load()
     t0 = time.time()
+    if args.eval:
+        return eval_only(lib, args)
     files = args.moments.split(",") if args.moments else []
     if files and all(os.path.exists(f) for f in files) and not args.gen_only:
         mats = [m for f in files for m in load_moments(f)]

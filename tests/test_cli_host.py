@@ -20,7 +20,8 @@ def test_unknown_mode_is_rejected():
 
 @pytest.mark.parametrize("args", [["--nmat", "-5", "--gen-only"], ["--nint", "0"],
                                   ["--epochs", "0"], ["--holdout", "1.5"],
-                                  ["--nmin", "10", "--nmax", "5"]])
+                                  ["--nmin", "10", "--nmax", "5"],
+                                  ["--eval", "w.bin", "--grid-min", "50", "--grid-max", "20"]])
 def test_slicer_cli_rejects_bad_sizes(args):
     """The training CLI checks its sizes before loading the library."""
     from pevsl_amd import slicer
