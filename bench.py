@@ -9,8 +9,9 @@ pEVSL_ChebLanNrBatch call solves all 20 slices as the columns of block SpMMs
 (FP32 filter in Clenshaw form at the autotuned block width; FP64 Lanczos
 coefficients, reorthogonalization, tridiagonal solves and Rayleigh-Ritz).
 The Lanczos basis is stored in FP64 (default; the 20 FP64 bases do not fit
-in HBM together, so the call runs in waves of 8, 8 and 4 slices,
-pEVSL_SetBatchWave) or, with --basis fp32, in FP32 in one wave.
+in HBM together, so the call runs in waves of 4, 8 and 8 slices in order
+of filter degree, pEVSL_SetBatchWave) or, with --basis fp32, in FP32 in one
+wave.
 
 The run has four parts, each reported in the JSON line:
 
@@ -478,8 +479,11 @@ def main():
         k = args.k or (max(64, int(np.mean(its)) // 2) if its else 1424)
         k = min(k, n - 2)
         sb = capi.StepBench()
-        # the slices of one wave: the lowest degrees (waves go in degree order)
-        wsl = sorted(range(ns), key=lambda s_: degs[s_])[:wave] if wave else list(range(ns))
+        # the slices of the first full wave (waves go in degree order, the
+        # remainder of ns / wave first)
+        rem = ns % wave if wave and ns > wave else 0
+        wsl = (sorted(range(ns), key=lambda s_: degs[s_])[rem:rem + wave] if wave
+               else list(range(ns)))
         nw_ = len(wsl)
         ivarr = np.ascontiguousarray(np.asarray([intvs[s_] for s_ in wsl],
                                                 dtype=np.float64).reshape(nw_, 4))

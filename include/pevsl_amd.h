@@ -160,8 +160,9 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs,
 
 /* Slices per wave of pEVSL_ChebLanNrBatch (0, the default: all slices in
  * one batch).  With wave > 0 a call with more slices solves them in
- * ceil(nslices / wave) batches of `wave` slices (the last takes the rest),
- * in order of filter degree, one after the other: peak device memory is
+ * ceil(nslices / wave) batches in order of filter degree, one after the
+ * other: first the nslices % wave lowest-degree slices (if any), then full
+ * batches of `wave` slices.  Peak device memory is
  * then that of the largest wave
  * (each slice's Lanczos basis grows to n x its step count, in 8-byte FP64
  * or 4-byte FP32 elements).  Outputs are per slice as for one batch. */

@@ -1656,9 +1656,11 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs, int *max
       bs.outputs(nevOut, lamo, Wo, reso, vinit);
       return rc;
     }
-    // waves of w slices (the last takes the rest), in order of filter degree
-    // so that the slices of one wave share filter groups of similar degree;
-    // full waves keep the filter groups at their full width
+    // waves of w slices in order of filter degree, so that the slices of one
+    // wave share filter groups of similar degree; the remainder (nslices % w)
+    // goes first, on the lowest degrees, where its narrower filter groups
+    // cost least (100^3 Laplacian, 20 slices in waves of 8: 4 + 8 + 8 took
+    // 89.7 s of filter against 91.0 s for 8 + 8 + 4)
     std::vector<int> order((size_t)nslices);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
@@ -1666,8 +1668,9 @@ int pEVSL_ChebLanNrBatch(pevsl_Data *pevsl, int nslices, double *intvs, int *max
     const int nw = (nslices + w - 1) / w;
     std::vector<pevsl_LanInfo> info((size_t)nslices, pevsl_LanInfo{});
     int rc = 0, pos = 0;
+    const int rem = nslices % w;
     for (int wi = 0; wi < nw; wi++) {
-      const int m = std::min(w, nslices - pos);
+      const int m = (wi == 0 && rem) ? rem : w;
       std::vector<double> iv((size_t)4 * m);
       std::vector<int> mi((size_t)m), nev((size_t)m);
       std::vector<pevsl_Polparams> pp((size_t)m);
