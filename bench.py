@@ -157,6 +157,19 @@ def roofline_of(ktimes, traffic, prefer=None, col16=False):
             "launches": t["launches"], "seconds": round(t["seconds"], 4)}
 
 
+def first_full_wave(degs, wave):
+    """Slices of the first full wave of pEVSL_ChebLanNrBatch with
+    pEVSL_SetBatchWave(wave): waves go in order of filter degree (stable for
+    equal degrees), the remainder of len(degs) / wave first; wave 0 (or at
+    least len(degs)) is one wave of all slices."""
+    ns = len(degs)
+    if not wave or wave >= ns:
+        return list(range(ns))
+    order = sorted(range(ns), key=lambda s_: degs[s_])
+    rem = ns % wave
+    return order[rem:rem + wave]
+
+
 def fp64_model(meas, slices):
     """Per-slice FP64 time T = cf * deg * (its + 1) + cr * its^2 / 2 + co * its,
     cf from the measured filter times, cr from the Gram-Schmidt/update times
@@ -479,11 +492,7 @@ def main():
         k = args.k or (max(64, int(np.mean(its)) // 2) if its else 1424)
         k = min(k, n - 2)
         sb = capi.StepBench()
-        # the slices of the first full wave (waves go in degree order, the
-        # remainder of ns / wave first)
-        rem = ns % wave if wave and ns > wave else 0
-        wsl = (sorted(range(ns), key=lambda s_: degs[s_])[rem:rem + wave] if wave
-               else list(range(ns)))
+        wsl = first_full_wave(degs, wave)
         nw_ = len(wsl)
         ivarr = np.ascontiguousarray(np.asarray([intvs[s_] for s_ in wsl],
                                                 dtype=np.float64).reshape(nw_, 4))

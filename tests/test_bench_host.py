@@ -56,3 +56,17 @@ def test_roofline_picks_the_kernel_with_most_time():
     r = bench.roofline_of(kt, {"k_a<float,8>": 2e8})
     assert r["kernel"] == "k_a<float,8>" and r["traffic"] == 2e8
     assert abs(r["achieved"] - 1875.0) < 1e-6 and abs(r["frac"] - 1875.0 / 8000.0) < 1e-4
+
+
+def test_first_full_wave_follows_the_library_order():
+    """The bench times the first full wave: degree order, remainder first
+    (as pEVSL_ChebLanNrBatch schedules waves)."""
+    degs = [300, 226, 456, 240, 410, 253, 266, 279, 292, 304, 316, 328, 340, 352, 365, 376,
+            388, 399, 422, 433]
+    w = bench.first_full_wave(degs, 8)
+    order = sorted(range(20), key=lambda s: degs[s])
+    assert w == order[4:12]
+    assert bench.first_full_wave(degs, 0) == list(range(20))
+    assert bench.first_full_wave(degs, 20) == list(range(20))
+    assert bench.first_full_wave(degs, 5) == order[0:5]   # no remainder
+    assert bench.first_full_wave([3, 1, 2], 2) == [2, 0]  # remainder 1: slice 1 goes first
