@@ -109,6 +109,11 @@ struct KernTimers {
 // block kernels.
 constexpr int kUnevenRow = 64;
 
+// Highest filter degree a batched FP32 group runs in Clenshaw form; above it
+// the group runs the three-term form (same vector passes with y updated every
+// other degree, smaller rounding error at high degree).
+constexpr int kClenMaxDeg = 400;
+
 // Per-instance device context (attached to pevsl_Data.dev).
 struct Ctx {
   int device = 0;

@@ -387,7 +387,7 @@ k_cheb_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
               const int *__restrict__ ja, const short *__restrict__ jo, const T *__restrict__ a,
               const T *__restrict__ vk, const T *__restrict__ vkm1,
               T *__restrict__ vkp1, T *__restrict__ y, T cc, T t,
-              const double *__restrict__ mus, int k, int nact) {
+              const double *__restrict__ mus, int k, int nact, int ymode) {
 #pragma clang fp contract(off)
   __shared__ T as[kStreamNnz];
   __shared__ int js[kStreamNnz];
@@ -420,7 +420,7 @@ k_cheb_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
   }
   T v[B], vm[B], yy[B], w[B];
   load_row<T, B>(vk + (size_t)r * B, v);
-  load_row<T, B>(y + (size_t)r * B, yy);
+  if (ymode != kYSkip) load_row<T, B>(y + (size_t)r * B, yy);
   if (k == 1) {
 #pragma unroll
     for (int c = 0; c < B; c++) {
@@ -435,11 +435,17 @@ k_cheb_blkrow(const int *__restrict__ rowblk, const int *__restrict__ ia,
       w[c] = acc[c] + (-cc) * v[c];
       w[c] = w[c] * t;
       w[c] = w[c] + (T)(-1) * vm[c];
-      if (c < nact) yy[c] = yy[c] + (T)mus[(size_t)k * B + c] * w[c];
+      if (c < nact) {
+        // kYPair: the term of degree k-1 (v, this step's input) that the
+        // previous step skipped, then degree k: the two updates of two plain
+        // steps, in their order
+        if (ymode == kYPair) yy[c] = yy[c] + (T)mus[(size_t)(k - 1) * B + c] * v[c];
+        if (ymode != kYSkip) yy[c] = yy[c] + (T)mus[(size_t)k * B + c] * w[c];
+      }
     }
   }
   store_row<T, B>(vkp1 + (size_t)r * B, w);
-  store_row<T, B>(y + (size_t)r * B, yy);
+  if (ymode != kYSkip) store_row<T, B>(y + (size_t)r * B, yy);
 }
 
 // One Clenshaw step of the filter on B interleaved columns (row per thread,
@@ -668,7 +674,7 @@ struct EpiCheb {
   T *vkp1, *y;
   T cc, t;
   const double *mus;
-  int k, nact;
+  int k, nact, ymode;
   __device__ __forceinline__ void operator()(int r, int piece,
                                              const T (&acc)[Pieces<T, B>::P]) const {
 #pragma clang fp contract(off)
@@ -676,7 +682,7 @@ struct EpiCheb {
     const size_t o = (size_t)r * B + piece * P;
     T v[P], yy[P], w[P];
     ld_piece<T, P>(vk + o, v);
-    ld_piece<T, P>(y + o, yy);
+    if (ymode != kYSkip) ld_piece<T, P>(y + o, yy);
     if (k == 1) {
 #pragma unroll
       for (int e = 0; e < P; e++) {
@@ -694,11 +700,14 @@ struct EpiCheb {
         w[e] = acc[e] + (-cc) * v[e];
         w[e] = w[e] * t;
         w[e] = w[e] + (T)(-1) * vm[e];
-        if (c < nact) yy[e] = yy[e] + (T)mus[(size_t)k * B + c] * w[e];
+        if (c < nact) {
+          if (ymode == kYPair) yy[e] = yy[e] + (T)mus[(size_t)(k - 1) * B + c] * v[e];
+          if (ymode != kYSkip) yy[e] = yy[e] + (T)mus[(size_t)k * B + c] * w[e];
+        }
       }
     }
     st_piece<T, P>(vkp1 + o, w);
-    st_piece<T, P>(y + o, yy);
+    if (ymode != kYSkip) st_piece<T, P>(y + o, yy);
   }
 };
 
@@ -1787,7 +1796,7 @@ int bal_level(int elem, int B) {
 template <typename T>
 void cheb_step_bal(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
                    T *y, double cc, double t, const double *mus, int k,
-                   int nact, hipStream_t s) {
+                   int nact, hipStream_t s, int ymode) {
   if (A.n == 0 || nact <= 0) return;
   const int lv = bal_level((int)sizeof(T), B);
   if (lv > 4 || !A.rowblk_q[lv] || A.nrowblk_q[lv] == 0) return;
@@ -1795,7 +1804,7 @@ void cheb_step_bal(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
   switch (B) {
 #define CASE(BB)                                                                         \
   case BB: {                                                                             \
-    const EpiCheb<T, BB> epi{vk, vkm1, vkp1, y, (T)cc, (T)t, mus, k, nact};              \
+    const EpiCheb<T, BB> epi{vk, vkm1, vkp1, y, (T)cc, (T)t, mus, k, nact, ymode};       \
     hipLaunchKernelGGL((k_blk_bal<T, BB, EpiCheb<T, BB>>), dim3(A.nrowblk_q[lv]),        \
                        dim3(kThreads), 0, s, A.rowblk_q[lv], A.ia, A.ja, a, vk, epi);    \
     break;                                                                               \
@@ -1834,10 +1843,10 @@ void clen_step_bal(const DevCsr &A, int B, const T *b1, const T *b2, const T *v,
 }
 template void cheb_step_bal<double>(const DevCsr &, int, const double *, const double *,
                                     double *, double *, double, double, const double *, int,
-                                    int, hipStream_t);
+                                    int, hipStream_t, int);
 template void cheb_step_bal<float>(const DevCsr &, int, const float *, const float *, float *,
                                    float *, double, double, const double *, int, int,
-                                   hipStream_t);
+                                   hipStream_t, int);
 template void clen_step_bal<double>(const DevCsr &, int, const double *, const double *,
                                     const double *, double *, double, double, const double *,
                                     int, hipStream_t);
@@ -1902,7 +1911,7 @@ void cheb_step_blk(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
 template <typename T>
 void cheb_step_blkrow(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
                       T *y, double cc, double t, const double *mus, int k,
-                      int nact, hipStream_t s) {
+                      int nact, hipStream_t s, int ymode) {
   if (A.n == 0 || nact <= 0 || A.nrowblk == 0) return;
   const T *a = std::is_same<T, double>::value ? (const T *)A.a64
                                               : (const T *)A.a32;
@@ -1912,11 +1921,11 @@ void cheb_step_blkrow(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp
     if (A.jo16)                                                                         \
       hipLaunchKernelGGL((k_cheb_blkrow<T, BB, true>), dim3(A.nrowblk), dim3(kThreads), 0, \
                          s, A.rowblk, A.ia, A.ja, A.jo16, a, vk, vkm1, vkp1, y, (T)cc,   \
-                         (T)t, mus, k, nact);                                           \
+                         (T)t, mus, k, nact, ymode);                                    \
     else                                                                                \
       hipLaunchKernelGGL((k_cheb_blkrow<T, BB, false>), dim3(A.nrowblk), dim3(kThreads), 0, \
                          s, A.rowblk, A.ia, A.ja, A.jo16, a, vk, vkm1, vkp1, y, (T)cc,   \
-                         (T)t, mus, k, nact);                                           \
+                         (T)t, mus, k, nact, ymode);                                    \
     break;
     CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32)
 #undef CASE
@@ -1988,10 +1997,10 @@ template void clen_step_seg<double>(const DevCsr &, int, const double *, const d
 
 template void cheb_step_blkrow<double>(const DevCsr &, int, const double *,
                                        const double *, double *, double *, double,
-                                       double, const double *, int, int, hipStream_t);
+                                       double, const double *, int, int, hipStream_t, int);
 template void cheb_step_blkrow<float>(const DevCsr &, int, const float *, const float *,
                                       float *, float *, double, double, const double *,
-                                      int, int, hipStream_t);
+                                      int, int, hipStream_t, int);
 
 template void cheb_step_blk<double>(const DevCsr &, int, const double *,
                                     const double *, double *, double *, double,

@@ -76,12 +76,23 @@ void cheb_step_blk(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
                    T *y, double cc, double t, const double *mus, int k,
                    int nact, hipStream_t s);
 
+// y-update modes of the three-term block steps (cheb_step_blkrow,
+// cheb_step_bal; k >= 2 for kYPair):
+//   kYPlain  y += mu_k vkp1 (k == 1: y = mu_0 vk + mu_1 vkp1)
+//   kYSkip   y is neither read nor written (its term waits for the next step)
+//   kYPair   y += mu_{k-1} vk, then y += mu_k vkp1: the updates of a skipped
+//            step and this one, in their order, for columns c < nact (active
+//            at degree k-1; mus is zero past each column's degree)
+// Skip and pair steps alternate, so y costs one read and one write per two
+// degrees, with the results of plain steps bit for bit.
+constexpr int kYPlain = 0, kYSkip = 2, kYPair = 1;
+
 // Same with one thread per row (row blocks of the CSR-stream partition,
 // a/ja staged in LDS, B-wide 16-byte row gathers); B <= 32.
 template <typename T>
 void cheb_step_blkrow(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
                       T *y, double cc, double t, const double *mus, int k,
-                      int nact, hipStream_t s);
+                      int nact, hipStream_t s, int ymode = kYPlain);
 
 // Clenshaw step on B interleaved columns (row per thread, B <= 32):
 //   out = mu_k v + t (A b1 - cc b1) - b2,  mu_k per column = mus[k*B + c];
@@ -112,7 +123,7 @@ void clen_step_seg(const DevCsr &A, int Bp, const T *b1, const T *b2, const T *v
 template <typename T>
 void cheb_step_bal(const DevCsr &A, int B, const T *vk, const T *vkm1, T *vkp1,
                    T *y, double cc, double t, const double *mus, int k,
-                   int nact, hipStream_t s);
+                   int nact, hipStream_t s, int ymode = kYPlain);
 template <typename T>
 void clen_step_bal(const DevCsr &A, int B, const T *b1, const T *b2, const T *v, T *out,
                    double cc, double t, const double *mus, int k, hipStream_t s);

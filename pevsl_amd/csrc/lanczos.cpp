@@ -90,10 +90,11 @@ double clen_seg_launch_bytes(const DevCsr &A, int elem, int seg, int d, int nseg
   return b;
 }
 
-size_t filter_launch_bytes(const DevCsr &A, int elem, int ncol, bool first, int idx) {
+// three-term step: passes = vector passes per column (5: vk gathered, vkm1,
+// vkp1, y read and written; 3 on the first degree, or with y skipped)
+size_t filter_launch_bytes(const DevCsr &A, int elem, int ncol, int passes, int idx) {
   const double vec = (double)A.n * elem;
-  return (size_t)((double)A.nnz * (elem + idx) + 4.0 * (A.n + 1) +
-                  (first ? 3.0 : 5.0) * vec * ncol);
+  return (size_t)((double)A.nnz * (elem + idx) + 4.0 * (A.n + 1) + passes * vec * ncol);
 }
 
 // name of a filter kernel for the per-kernel timers
@@ -150,7 +151,7 @@ static void cheb_filter_f32(Ctx &c, const pevsl_Polparams *pol, const float *x32
                            mu[k], k == 1, lanes, s);
     vkm1 = vk;
     vk = vkp1;
-    bytes += (double)filter_launch_bytes(A, 4, 1, k == 1, stream ? index_bytes(A) : 4);
+    bytes += (double)filter_launch_bytes(A, 4, 1, k == 1 ? 3 : 5, stream ? index_bytes(A) : 4);
   }
   if (tok >= 0) c.kt.end(s, tok, m, bytes);
   c.kstats.filter_bytes += bytes;
@@ -215,7 +216,7 @@ void cheb_filter_col(Ctx &c, const pevsl_Polparams *pol, const double *dv,
       vkm1 = vk;
       vk = vkp1;
       c.kstats.filter_bytes +=
-          (double)filter_launch_bytes(A, 8, 1, k == 1, stream ? index_bytes(A) : 4);
+          (double)filter_launch_bytes(A, 8, 1, k == 1 ? 3 : 5, stream ? index_bytes(A) : 4);
     }
     if (tok >= 0) c.kt.end(s, tok, m, c.kstats.filter_bytes - bytes0);
   } else {
@@ -374,7 +375,7 @@ class BatchSolver {
   void set_active(const std::vector<int> &act);
   void filter_step(const std::vector<int> &act, int k);
   void filter_block(const std::vector<int> &grp, int k);
-  int group_width(int g, int &kind) const;
+  int group_width(const std::vector<int> &grp, int &kind) const;
   char *blocks_for(int B);
   void stage_mus(const std::vector<int> &grp, int B, DBuf<double> &dst);
   void stage_colptr(const std::vector<int> &grp, int B, int k, DBuf<void *> &dst);
@@ -510,9 +511,20 @@ void BatchSolver::free_basis(int s) {
 // narrowest width that holds the group (groups never exceed the tuned width:
 // a width-8 block holding 4 columns would stream twice the vector bytes it
 // needs); the segmented kernel takes any multiple of its 16-byte piece.
-int BatchSolver::group_width(int g, int &kind) const {
+int BatchSolver::group_width(const std::vector<int> &grp, int &kind) const {
+  const int g = (int)grp.size();
   const int seg = c_.filter_prec == PEVSL_PREC_FP32 ? 4 : 2;
   kind = block_kind();
+  // Clenshaw sums lose accuracy with the degree faster than the three-term
+  // recurrence: on the n = 2e6 power-law matrix at degree 508 the FP32
+  // Clenshaw filter left accepted residuals up to 8.6e-5 (threshold 1.1e-4)
+  // and missed one of 32 eigenvalues; the three-term filter, 1.1e-5 and all
+  // 32.  Above kClenMaxDeg a group runs the three-term form of its kernel.
+  if (kind == 2 || kind == 4 || kind == 5) {
+    int degmax = 0;
+    for (int s : grp) degmax = std::max(degmax, S_[(size_t)s].pol->deg);
+    if (degmax > kClenMaxDeg) kind = (kind == 4 || (kind == 5 && c_.A->max_row > kUnevenRow)) ? 3 : 1;
+  }
   int B = 1;
   if (kind == 5)
     B = (g + seg - 1) / seg * seg;
@@ -563,7 +575,7 @@ void BatchSolver::stage_colptr(const std::vector<int> &grp, int B, int k, DBuf<v
 
 void BatchSolver::filter_block(const std::vector<int> &grp, int k) {
   int kind;
-  const int B = group_width((int)grp.size(), kind);
+  const int B = group_width(grp, kind);
   std::vector<int> key(grp);
   key.push_back(B);
   if (key != mus_key_) {
@@ -700,6 +712,7 @@ void BatchSolver::enqueue_block(const std::vector<int> &grp, int B, int kind,
     int nb = 0;
     const long tok = degmax > 0 && !capture_ ? c_.kt.begin(c_.stream, kname.c_str()) : -1;
     double kbytes = 0.0;
+    int nprev = g;  // columns whose series reaches degree d - 1
     for (int d = 1; d <= degmax; d++) {
       int nact = 0;
       while (nact < g && degs[(size_t)nact] >= d) nact++;
@@ -710,17 +723,37 @@ void BatchSolver::enqueue_block(const std::vector<int> &grp, int B, int kind,
         vkp1 = bufs[nb];
       }
       nb = (nb + 1) % 3;
+      // the row kernels update y every other degree (skip, then pair: the
+      // same additions in the same order, one pass over y per two degrees);
+      // the last degree updates it alone when it would be skipped
+      int ymode = kYPlain, ncol = nact, passes = d == 1 ? 3 : 5;
+      if (rowk && d > 1) {
+        if (d % 2 == 1) {
+          ymode = kYPair;
+          ncol = nprev;
+        } else if (d < degmax) {
+          ymode = kYSkip;
+          passes = 3;
+        }
+      }
       if (bal)
         cheb_step_bal<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd, musp, d,
-                         nact, c_.stream);
+                         ncol, c_.stream, ymode);
       else if (rowk)
         cheb_step_blkrow<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd,
-                            musp, d, nact, c_.stream);
+                            musp, d, ncol, c_.stream, ymode);
       else
         cheb_step_blk<T>(A, B, vk, vkm1, vkp1, y, cc, d == 1 ? 1.0 / dd : 2.0 / dd,
                          musp, d, nact, c_.stream);
+      nprev = nact;
+      // pair steps count y over the columns of degree d - 1 (a product of
+      // its own over the columns of degree d); skip steps touch no y
+      const int ib = rowk && !bal ? index_bytes(A) : 4;  // only the row kernel reads jo16
       const double lb =
-          (double)filter_launch_bytes(A, (int)elem, nact, d == 1, rowk && !bal ? index_bytes(A) : 4);
+          ymode == kYPair
+              ? (double)filter_launch_bytes(A, (int)elem, nact, 3, ib) +
+                    2.0 * (double)n_ * (double)elem * ncol
+              : (double)filter_launch_bytes(A, (int)elem, nact, passes, ib);
       kbytes += lb;
       c_.kstats.filter_bytes += lb;
       c_.kstats.filter_steps += nact;
@@ -799,7 +832,7 @@ void BatchSolver::filter_step(const std::vector<int> &act, int k) {
     int bmax = 1;
     for (const std::vector<int> &grp : groups) {
       int kind;
-      if (grp.size() > 1) bmax = std::max(bmax, group_width((int)grp.size(), kind));
+      if (grp.size() > 1) bmax = std::max(bmax, group_width(grp, kind));
     }
     blocks_for(bmax);
     for (auto &gg : graphs_) gg->used = false;
@@ -828,7 +861,7 @@ void BatchSolver::filter_step(const std::vector<int> &act, int k) {
 // application.  Per step only the column pointers are restaged.
 void BatchSolver::filter_group_graph(const std::vector<int> &grp, int k) {
   int kind;
-  const int B = group_width((int)grp.size(), kind);
+  const int B = group_width(grp, kind);
   std::vector<int> key(grp);
   key.push_back(B);
   key.push_back(kind);

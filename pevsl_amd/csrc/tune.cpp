@@ -66,17 +66,21 @@ double time_blk(Ctx &c, int B, int reps, DBuf<double> &mus, int kind) {
   SegTops tops{};
   for (int p = 0; p < kMaxSegs; p++) tops.top[p] = 100;  // every piece forms the product
   const int nseg = B * (int)sizeof(T) / 16;
+  // the three-term row kernels alternate skip and pair steps, as a batched
+  // solve runs them (on average four vector passes per degree)
+  int parity = 0;
   auto step = [&](const T *vk, const T *vkm1, T *vkp1) {
+    const int ymode = (parity++ & 1) ? kYPair : kYSkip;
     if (kind == 5)
       clen_step_seg<T>(A, B, vk, vkm1, y, vkp1, 6.0, 1.0 / 3, mus.p, 2, nseg, tops, c.stream);
     else if (kind == 4)
       clen_step_bal<T>(A, B, vk, vkm1, y, vkp1, 6.0, 1.0 / 3, mus.p, 2, c.stream);
     else if (kind == 3)
-      cheb_step_bal<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
+      cheb_step_bal<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream, ymode);
     else if (kind == 2)
       clen_step_blkrow<T>(A, B, vk, vkm1, y, vkp1, 6.0, 1.0 / 3, mus.p, 2, c.stream);
     else if (kind == 1)
-      cheb_step_blkrow<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
+      cheb_step_blkrow<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream, ymode);
     else
       cheb_step_blk<T>(A, B, vk, vkm1, vkp1, y, 6.0, 1.0 / 6, mus.p, 2, B, c.stream);
   };
@@ -215,9 +219,10 @@ int pEVSL_BenchBlock(pevsl_Data *p, int B, int reps, double *sec, double *bytes)
   PV_HIP(hipMemcpy(mus.p, mh.data(), sizeof(double) * mh.size(), hipMemcpyHostToDevice));
   *sec = f32 ? time_blk<float>(c, B, reps, mus, kind) : time_blk<double>(c, B, reps, mus, kind);
   const int idx = (kind == 1 || kind == 2) ? index_bytes(*c.A) : 4;
-  *bytes = (kind == 2 || kind == 4 || kind == 5)
-               ? step_bytes(*c.A, elem, B, idx) - (double)c.A->n * B * elem
-               : step_bytes(*c.A, elem, B, idx);
+  // four vector passes per degree for all but the lane-per-(row, column)
+  // kernel (Clenshaw; three-term row kernels with y every other degree)
+  *bytes = kind != 0 ? step_bytes(*c.A, elem, B, idx) - (double)c.A->n * B * elem
+                     : step_bytes(*c.A, elem, B, idx);
   return 0;
 }
 
