@@ -128,15 +128,18 @@ int pEVSL_SetSpmvTuning(pevsl_Data *pevsl, int lanes_per_row, int block_width);
 
 /* Block SpMM kernel of batched solves: 1 = one thread per row with B-wide
  * row gathers (B <= 32) and the reference's three-term recurrence with an
- * accumulator (5 vector passes per degree), 2 = the same kernel in Clenshaw
- * form (4 passes), 3 / 4 = the nnz-balanced forms of 1 / 2 (products formed
- * in parallel over nonzeros and staged in LDS, long rows reduced by a whole
- * workgroup: for uneven row lengths), 0 = one lane per (row, column),
- * 5 = the Clenshaw form with one thread per (row, 16-byte piece of columns):
- * up to 64 FP32 columns share one pass over the matrix, each piece active
- * from its own top degree (FP32 filter only; 1 on FP64).
+ * accumulator updated every other degree (4 vector passes per degree on
+ * average, the sums of one update per degree bit for bit), 2 = the same
+ * kernel in Clenshaw form (4 passes), 3 / 4 = the nnz-balanced forms of
+ * 1 / 2 (products formed in parallel over nonzeros and staged in LDS, long
+ * rows reduced by a whole workgroup: for uneven row lengths), 0 = one lane
+ * per (row, column), 5 = the Clenshaw form with one thread per (row, 16-byte
+ * piece of columns): up to 64 FP32 columns share one pass over the matrix,
+ * each piece active from its own top degree (FP32 filter only; 1 on FP64).
  * Default (-1): 2 for an FP32 filter, 1 for FP64, or 4 / 3 when the
- * matrix has a row longer than 64 entries. */
+ * matrix has a row longer than 64 entries.  A group of slices whose top
+ * degree exceeds 400 runs the three-term form of its kernel (2 -> 1,
+ * 4 -> 3, 5 -> 1 or 3): Clenshaw sums lose accuracy at high degree. */
 int pEVSL_SetBlockKernel(pevsl_Data *pevsl, int rowkernel);
 
 /* Eigenvalues >= thr of the symmetric tridiagonal matrix (diag[0..n-1],
