@@ -80,7 +80,7 @@ int pEVSL_SetMixedTol(pevsl_Data *pevsl, double tol);
 
 /* Ritz stage: 0 = the reference's per-vector Ritz pairs (SRC/cheblanNr.c:
  * 339-413); 1 = FP64 Rayleigh-Ritz refinement of the selected Ritz vectors
- * with A ((U^T A U) c = theta (U^T U) c, DGEMM Gram products + SYGVD) before
+ * with A ((U^T A U) c = theta (U^T U) c, Gram products on FP64 MFMA + SYGVD) before
  * the same acceptance test; -1 (default) = refinement only with the FP32
  * filter.  The refinement separates eigenvectors that the filter resolves
  * only to its own precision. */
